@@ -1,0 +1,180 @@
+"""Write the golden vectors under tests/golden/ from the reference itself.
+
+Run in the development container (where the reference sources exist):
+    make -f oracle/Makefile.ref && python tests/golden/make_golden.py
+
+For every configuration below it
+  1. builds the synthetic initial state and writes it as a mode-2 file
+     (readenc.f90:104-113);
+  2. runs oracle/_ref/ref_harness -- the reference's own Fortran routines
+     (steprk.f90, steppc.f90, poten.f90, fft.f90 + MKL FFTW3, potenimp.f90,
+     energy.f90, ...) driven in the order of DFT4He3d.f90:834-852;
+  3. stores inputs and outputs in <name>.npz.  Outputs are stored on three
+     orthogonal planes through the box (the full fields are reproduced by any
+     correct implementation, and three planes keep the fixtures small).
+It also extracts the centred rows of the reference's finite-difference table
+(blockds.f) for M = 3..25, K = 1, 2 into fd_coeffs.json, and the two
+Tang-Toennies Xe-He values quoted in V_impur.f90:759-760 and the sample input.
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "4hetddft-isotropic_amd"))
+from hetd.initstate import KTOPS, woods_saxon_state, write_mode2  # noqa: E402
+
+REF = "/root/reference"
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+CONFIGS = {
+    # OT + solid penalty (the reference default), Xe impurity, zero halo,
+    # absorbing time constant along z, 12th-order stencils
+    "ot_xe_32": dict(nx=32, ny=32, nz=32, xmax=16.0, ymax=16.0, zmax=16.0, npd=13, icon=13,
+                     core="OT", lsolid=True, selec="Xe_He", r_cutoff=2.55, umax=2.7939365463e3,
+                     n4=100, seed=11, rimp=(0.5, -0.3, 0.2), vimp_aps=(0.3, 0.1, 2.0),
+                     niter=24, pener=4, tzmean=12.0),
+    # full Orsay-Trento (alpha_s term), periodic stencils, 8th order,
+    # non-cubic grid with radix-3 and radix-5 lengths
+    "otc_per_36x32x40": dict(nx=36, ny=32, nz=40, xmax=18.0, ymax=16.0, zmax=20.0, npd=9, icon=8,
+                             core="OTC", lsolid=False, selec="Xe_He", r_cutoff=2.55,
+                             umax=2.7939365463e3, n4=120, seed=5, rimp=(-1.0, 0.4, 0.0),
+                             vimp_aps=(0.0, -0.5, 1.5), niter=12, pener=4, tzmean=1000.0),
+    # odd lengths, no impurity field
+    "ot_null_45x30x27": dict(nx=45, ny=30, nz=27, xmax=18.0, ymax=12.0, zmax=10.8, npd=13, icon=13,
+                             core="OT", lsolid=True, selec="null", r_cutoff=2.0, umax=0.0,
+                             n4=50, seed=7, rimp=(0.0, 0.0, 0.0), vimp_aps=(0.0, 0.0, 0.0),
+                             niter=10, pener=5, tzmean=1000.0),
+}
+
+
+def planes(f):
+    """Three orthogonal central planes of a (nz, ny, nx) field."""
+    nz, ny, nx = f.shape
+    return np.concatenate([f[nz // 2, :, :].ravel(), f[:, ny // 2, :].ravel(), f[:, :, nx // 3].ravel()])
+
+
+def run_config(name, c, work):
+    psi = woods_saxon_state(c["nx"], c["ny"], c["nz"], c["xmax"], c["ymax"], c["zmax"], c["n4"], c["seed"])
+    vint = [v * KTOPS for v in c["vimp_aps"]]
+    den_in = os.path.join(work, "den.in")
+    write_mode2(den_in, psi, c["xmax"], c["ymax"], c["zmax"], c["rimp"], vint, header=name)
+    # the harness reads the exact bytes just written; read them back so the
+    # fixture holds the same psi as the reference consumed
+    from hetd.initstate import read_mode2
+    psi_r, _, r_r, v_r = read_mode2(den_in)
+    nml = os.path.join(work, "nml")
+    with open(nml, "w") as f:
+        f.write("&harness\n")
+        f.write(" nx=%d, ny=%d, nz=%d, xmax=%r, ymax=%r, zmax=%r,\n" % (
+            c["nx"], c["ny"], c["nz"], c["xmax"], c["ymax"], c["zmax"]))
+        f.write(" npd=%d, icon=%d, core4='%s', lsolid=%s,\n" % (
+            c["npd"], c["icon"], c["core"], ".true." if c["lsolid"] else ".false."))
+        f.write(" selec='%s', r_cutoff=%r, umax=%r, mimpur=131.29d0,\n" % (c["selec"], c["r_cutoff"], c["umax"]))
+        f.write(" deltatps=5.d-4, niter=%d, pener=%d, filedenin='den.in', outdir='%s',\n" % (
+            c["niter"], c["pener"], work))
+        f.write(" tzmean=%r, n4=%d\n/\n" % (c["tzmean"], c["n4"]))
+    env = dict(os.environ, MKL_THREADING_LAYER="SEQUENTIAL", OMP_NUM_THREADS="1")
+    with open(nml) as fin:
+        subprocess.run([HARNESS], stdin=fin, cwd=work, env=env, check=True,
+                       stdout=subprocess.DEVNULL)
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    n, nh = nx * ny * nz, (nx // 2 + 1) * ny * nz
+    raw = np.fromfile(os.path.join(work, "init.bin"))
+    shp, shph = (nz, ny, nx), (nz, ny, nx // 2 + 1)
+    off = 0
+    init = {}
+    for key in ("pot4", "dencg", "delj4", "ccor", "dxden", "dyden", "dzden", "uimp"):
+        init[key] = raw[off:off + n].reshape(shp)
+        off += n
+    init["fvlj"] = raw[off:off + nh].reshape(shph)
+    off += nh
+    init["wcgk"] = raw[off:off + nh].reshape(shph)
+    off += nh
+    init["fden"] = raw[off:off + 2 * nh].view(np.complex128).reshape(shph)
+    off += 2 * nh
+    if c["lsolid"]:
+        init["penalty"] = raw[off:off + n].reshape(shp)
+        off += n
+    if c["core"] == "OTC":
+        init["ualphas"] = raw[off:off + n].reshape(shp)
+        off += n
+        init["denalf"] = raw[off:off + n].reshape(shp)
+        off += n
+    scal = raw[off:]
+    steps = np.fromfile(os.path.join(work, "steps.bin")).reshape(-1, 12)
+    energy = np.fromfile(os.path.join(work, "energy.bin")).reshape(-1, 11)
+    final = np.fromfile(os.path.join(work, "final.bin")).view(np.complex128).reshape(shp)
+    out = dict(psi0=psi_r, rimp0=r_r, vimp0=v_r, steps=steps, energy=energy,
+               init_energy=scal[:9], init_aimp=scal[9:12], final_planes=planes(final))
+    for key, val in init.items():
+        if key in ("fvlj", "wcgk", "fden"):
+            out["init_" + key + "_planes"] = planes(val)
+        else:
+            out["init_" + key + "_planes"] = planes(val)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    cfg = dict(c)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(cfg, f, indent=1)
+    print(name, "ok", steps.shape, energy.shape)
+
+
+def fd_table():
+    """Centred rows of the reference table (blockds.f DATA A(25792)), expanded
+    with the symmetry of newder.f:99-128."""
+    a = {}
+    with open(os.path.join(REF, "blockds.f")) as f:
+        text = f.read()
+    for m in re.finditer(r"DATA\s*\(A\(I\),I=\s*(\d+),\s*(\d+)\)/(.*?)/", text, re.S):
+        i0, i1 = int(m.group(1)), int(m.group(2))
+        body = re.sub(r"\n\s{5}&", " ", m.group(3))
+        vals = [float(v.replace("D", "E")) for v in re.findall(r"[-+]?\d\.\d+D[-+]\d+", body)]
+        assert len(vals) == i1 - i0 + 1, (i0, i1, len(vals))
+        for k, v in enumerate(vals):
+            a[i0 + k] = v
+    out = {}
+    for M in range(3, 26, 2):
+        ji = 1
+        for i in range(3, M - 1, 2):
+            ji += (i * i + 1) * (i - 2) // 2 + i
+        m2, m21o2 = M * M, (M * M + 1) // 2
+        for K in (1, 2):
+            jiK = ji + m21o2 * (K - 1)
+            sig = 1.0 - 2.0 * (K % 2)
+            dkf = [0.0] * (m2 + 1)
+            for j in range(1, m21o2):
+                dkf[j] = a[jiK - 1 + j]
+                dkf[m2 - j + 1] = sig * a[jiK - 1 + j]
+            dkf[m21o2] = a[jiK - 1 + m21o2]
+            ko2 = M // 2
+            row = [dkf[ko2 * M + j] for j in range(1, M + 1)]
+            if K == M - 1:  # full matrix stored, every row the same (newder.f:116-126)
+                row = [a[ji + m21o2 * (K - 1) + t] for t in range(M)]
+            out["%d_%d" % (M, K)] = row
+    with open(os.path.join(HERE, "fd_coeffs.json"), "w") as f:
+        json.dump(out, f, indent=0)
+    print("fd_coeffs.json", len(out))
+
+
+def main():
+    fd_table()
+    kat = {"V_Xe_He": [[2.0, 21450.4867067095], [2.55, 2793.9365463]],
+           "b4": -718.99, "source": "V_impur.f90:759-760, input_files sample umax, modules.f90:212"}
+    with open(os.path.join(HERE, "kat.json"), "w") as f:
+        json.dump(kat, f, indent=1)
+    names = sys.argv[1:] or list(CONFIGS)
+    for name in names:
+        with tempfile.TemporaryDirectory() as work:
+            run_config(name, CONFIGS[name], work)
+
+
+if __name__ == "__main__":
+    main()
