@@ -1,0 +1,927 @@
+// api.hip -- the C ABI of include/hetd.h: device context, call sequencing of
+// poten / steprk / steppc / energy on one HIP stream, and host copies.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "../../include/hetd.h"
+#include "hetd_kernels.h"
+#include "setup.hpp"
+
+using namespace hetd;
+
+struct hetd_ctx {
+    hetd_params p{};
+    int nx = 0, ny = 0, nz = 0, nxh = 0;
+    long n = 0, nh = 0;
+    double hx = 0, hy = 0, hz = 0, dxyz = 0, mAg = 0, renor = 0;
+    bool otc = false;
+    hipStream_t st = nullptr;
+    std::string err;
+    std::vector<void*> allocs;
+    // grid tables
+    double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_tcx = nullptr, *d_tcy = nullptr, *d_tcz = nullptr;
+    double2* d_tw[3] = {nullptr, nullptr, nullptr};
+    float2* d_tw32[3] = {nullptr, nullptr, nullptr};
+    FftAxis ax[3];
+    StencilCoef sc{};
+    Grid grid{};
+    // state (psi family rotates by pointer)
+    double2 *psi = nullptr, *psiold[3] = {}, *spare = nullptr, *hpsiold[2] = {}, *pc = nullptr, *q = nullptr;
+    double *den = nullptr, *pot4 = nullptr, *dencg = nullptr, *delj4 = nullptr, *tmp = nullptr;
+    double2 *A = nullptr, *B = nullptr, *Cs = nullptr;
+    double *fvlj = nullptr, *wcgk = nullptr, *kalfs = nullptr;
+    // alpha_s term (OTC)
+    double *dxden = nullptr, *dyden = nullptr, *dzden = nullptr, *denalf = nullptr, *falfs = nullptr;
+    double *intx = nullptr, *inty = nullptr, *intz = nullptr, *ualphas = nullptr;
+    double *s1 = nullptr, *s2 = nullptr, *s3 = nullptr, *s4 = nullptr, *s5 = nullptr, *s6 = nullptr;
+    double* potion = nullptr;
+    int npot = 0;
+    double delinter = 0;
+    ImpState* imp = nullptr;
+    double* partial = nullptr;
+    double* red = nullptr;
+    int* guard = nullptr;
+    double* traj = nullptr;
+    long traj_cap = 0;
+    double* dtmp[2] = {nullptr, nullptr};  // scratch of hetd_deriv / hetd_fft_* (allocated on first use)
+    int ioldp[3] = {1, 2, 3}, ioldh[2] = {1, 2}, ioldr[3] = {1, 2, 3}, ioldv[3] = {1, 2, 3}, iolda[2] = {1, 2};
+    long iter_since_set = 0;
+    bool have_state = false;
+    // kernel-class timing (hetd_timing)
+    int tclass = 0;
+    std::vector<hipEvent_t> tev;
+    size_t tused = 0;
+    long tlaunch = 0;
+    double tbytes = 0;
+};
+
+// ------------------------------------------------------------------ helpers
+
+static int fail(hetd_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HCHK(c, expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail((c), HETD_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename P> static hipError_t dalloc(hetd_ctx* c, P** p, size_t bytes) {
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, bytes ? bytes : 16);
+    if (e == hipSuccess) {
+        c->allocs.push_back(v);
+        *p = reinterpret_cast<P*>(v);
+    }
+    return e;
+}
+
+static void fill_grid(hetd_ctx* c) {
+    Grid& g = c->grid;
+    g.nx = c->nx; g.ny = c->ny; g.nz = c->nz; g.n = c->n; g.dxyz = c->dxyz;
+    g.x = c->d_x; g.y = c->d_y; g.z = c->d_z;
+    g.tcx = c->d_tcx; g.tcy = c->d_tcy; g.tcz = c->d_tcz;
+    g.lambdah = c->p.Lambdah;
+    g.potion = c->potion; g.npot = c->npot; g.delinter = c->delinter;
+}
+
+
+// ------------------------------------------------------- kernel-class timing
+
+static void tbeg(hetd_ctx* c, int k) {
+    if (c->tclass != k) return;
+    while (c->tused + 2 > c->tev.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) { c->tclass = 0; return; }
+        c->tev.push_back(e);
+    }
+    (void)hipEventRecord(c->tev[c->tused], c->st);
+}
+
+static void tend(hetd_ctx* c, int k, double bytes) {
+    if (c->tclass != k) return;
+    (void)hipEventRecord(c->tev[c->tused + 1], c->st);
+    c->tused += 2;
+    c->tlaunch += 1;
+    c->tbytes += bytes;
+}
+
+// launches with their algorithmic HBM bytes (FP64: 8 B real, 16 B complex
+// per point; half spectra have nxh*ny*nz points)
+static hipError_t run_x(hetd_ctx* c, XPassArgs<double> x) {
+    const double n = (double)c->n, nh = (double)c->nh;
+    double b = 0;
+    switch (x.mode) {
+    case XMODE_R2C: b = 8 * n + 16 * nh; break;
+    case XMODE_C2R: b = 16 * nh + 8 * n; break;
+    case XMODE_CG_S: b = 16 * nh + 8 * n + 8 * n + 16 * nh; break;
+    default: b = 16 * nh + 24 * n + 8 * n + (x.rout2 ? 8 * n : 0); break;
+    }
+    tbeg(c, HETD_K_XPASS);
+    hipError_t e = launch_xpass(x, c->st);
+    tend(c, HETD_K_XPASS, b);
+    return e;
+}
+
+static hipError_t run_y(hetd_ctx* c, YPassArgs<double> y, int nbuf) {
+    tbeg(c, HETD_K_YPASS);
+    hipError_t e = launch_ypass(y, c->nz, nbuf, c->st);
+    tend(c, HETD_K_YPASS, 32.0 * c->nh * nbuf);
+    return e;
+}
+
+static hipError_t run_z(hetd_ctx* c, ZPassArgs<double> z, int nbuf) {
+    double per = 32.0;
+    if (z.mode == ZMODE_CONV) per = 16.0 + 24.0 * z.nk + (z.fwd_out ? 16.0 : 0.0);
+    tbeg(c, HETD_K_ZPASS);
+    hipError_t e = launch_zpass(z, nbuf, c->st);
+    tend(c, HETD_K_ZPASS, per * c->nh * nbuf);
+    return e;
+}
+
+static hipError_t run_update(hetd_ctx* c, const UpdateArgs& u, int* nb) {
+    double per = 144.0;
+    if (u.kind == STAGE_RK) per = u.rk_j == 0 ? 96.0 : 80.0;
+    tbeg(c, HETD_K_UPDATE);
+    hipError_t e = launch_update(u, c->st, nb);
+    tend(c, HETD_K_UPDATE, per * c->n);
+    return e;
+}
+
+static hipError_t run_force(hetd_ctx* c, const ForceArgs& f, int* nb) {
+    tbeg(c, HETD_K_FORCE);
+    hipError_t e = launch_force_partial(f, c->st, nb);
+    tend(c, HETD_K_FORCE, 8.0 * c->n);
+    return e;
+}
+
+// ---------------------------------------------------------------- FFT chains
+
+// r2c of a real device field into the half spectrum `out` (x, y, z passes)
+static int r2c_dev(hetd_ctx* c, const double* in, double2* out) {
+    XPassArgs<double> x{};
+    x.ax = c->ax[0]; x.nlines = (long)c->ny * c->nz; x.mode = XMODE_R2C; x.rin = in; x.cout = out;
+    HCHK(c, run_x(c, x));
+    YPassArgs<double> y{};
+    y.ax = c->ax[1]; y.nxh = c->nxh; y.inv = 0; y.buf[0] = out;
+    HCHK(c, run_y(c, y, 1));
+    ZPassArgs<double> z{};
+    z.ax = c->ax[2]; z.nxh = c->nxh; z.ny = c->ny; z.mode = ZMODE_FWD; z.in[0] = out; z.out[0][0] = out;
+    HCHK(c, run_z(c, z, 1));
+    return 0;
+}
+
+// c2r (times scale) of the half spectrum `in` (destroyed) into `out`
+static int c2r_dev(hetd_ctx* c, double2* in, double* out, double scale) {
+    ZPassArgs<double> z{};
+    z.ax = c->ax[2]; z.nxh = c->nxh; z.ny = c->ny; z.mode = ZMODE_INV; z.in[0] = in; z.out[0][0] = in;
+    HCHK(c, run_z(c, z, 1));
+    YPassArgs<double> y{};
+    y.ax = c->ax[1]; y.nxh = c->nxh; y.inv = 1; y.buf[0] = in;
+    HCHK(c, run_y(c, y, 1));
+    XPassArgs<double> x{};
+    x.ax = c->ax[0]; x.nlines = (long)c->ny * c->nz; x.mode = XMODE_C2R; x.cin = in; x.rout = out; x.scale = scale;
+    HCHK(c, run_x(c, x));
+    return 0;
+}
+
+// For nb real fields: r2c, multiply by the real half-spectrum kernel K, c2r
+// times renor (the fftfw / *K / fftbk pattern of term_alfa.f90:41-80).
+static int convolve_dev(hetd_ctx* c, int nb, const double* const* in, double2* const* work, double* const* out,
+                        const double* K) {
+    for (int b = 0; b < nb; ++b) {
+        XPassArgs<double> x{};
+        x.ax = c->ax[0]; x.nlines = (long)c->ny * c->nz; x.mode = XMODE_R2C; x.rin = in[b]; x.cout = work[b];
+        HCHK(c, run_x(c, x));
+    }
+    YPassArgs<double> y{};
+    y.ax = c->ax[1]; y.nxh = c->nxh; y.inv = 0;
+    for (int b = 0; b < nb; ++b) y.buf[b] = work[b];
+    HCHK(c, run_y(c, y, nb));
+    ZPassArgs<double> z{};
+    z.ax = c->ax[2]; z.nxh = c->nxh; z.ny = c->ny; z.mode = ZMODE_CONV; z.nk = 1; z.kern[0] = K;
+    for (int b = 0; b < nb; ++b) { z.in[b] = work[b]; z.out[b][0] = work[b]; }
+    HCHK(c, run_z(c, z, nb));
+    y.inv = 1;
+    HCHK(c, run_y(c, y, nb));
+    for (int b = 0; b < nb; ++b) {
+        XPassArgs<double> x{};
+        x.ax = c->ax[0]; x.nlines = (long)c->ny * c->nz; x.mode = XMODE_C2R; x.cin = work[b]; x.rout = out[b];
+        x.scale = c->renor;
+        HCHK(c, run_x(c, x));
+    }
+    return 0;
+}
+
+static int deriv_dev(hetd_ctx* c, int id, int iv, int cplx, const double* f, double* df) {
+    DerivArgs d{};
+    d.nx = c->nx; d.ny = c->ny; d.nz = c->nz; d.sc = c->sc; d.id = id; d.iv = iv; d.cplx = cplx; d.f = f; d.df = df;
+    HCHK(c, launch_deriv(d, c->st));
+    return 0;
+}
+
+// term_alfa (term_alfa.f90:28-113), then pot4 += ualphas (poten.f90:103-106)
+static int term_alfa_dev(hetd_ctx* c) {
+    const long n = c->n;
+    HCHK(c, launch_falfs(c->denalf, c->p.den0s, c->falfs, n, c->st));
+    int rc;
+    if ((rc = deriv_dev(c, 1, 1, 0, c->den, c->dxden))) return rc;
+    if ((rc = deriv_dev(c, 1, 2, 0, c->den, c->dyden))) return rc;
+    if ((rc = deriv_dev(c, 1, 3, 0, c->den, c->dzden))) return rc;
+    HCHK(c, launch_mul(c->falfs, c->dxden, c->s1, n, c->st));
+    HCHK(c, launch_mul(c->falfs, c->dyden, c->s2, n, c->st));
+    HCHK(c, launch_mul(c->falfs, c->dzden, c->s3, n, c->st));
+    {
+        const double* in[3] = {c->s1, c->s2, c->s3};
+        double2* wk[3] = {c->A, c->B, c->Cs};
+        double* out[3] = {c->intx, c->inty, c->intz};
+        if ((rc = convolve_dev(c, 3, in, wk, out, c->kalfs))) return rc;
+    }
+    HCHK(c, launch_dot3(c->dxden, c->intx, c->dyden, c->inty, c->dzden, c->intz, c->s1, n, c->st));
+    {
+        const double* in[1] = {c->s1};
+        double2* wk[1] = {c->A};
+        double* out[1] = {c->ualphas};
+        if ((rc = convolve_dev(c, 1, in, wk, out, c->kalfs))) return rc;
+    }
+    HCHK(c, launch_mul(c->falfs, c->intx, c->s1, n, c->st));
+    HCHK(c, launch_mul(c->falfs, c->inty, c->s2, n, c->st));
+    HCHK(c, launch_mul(c->falfs, c->intz, c->s3, n, c->st));
+    if ((rc = deriv_dev(c, 1, 1, 0, c->s1, c->s4))) return rc;
+    if ((rc = deriv_dev(c, 1, 2, 0, c->s2, c->s5))) return rc;
+    if ((rc = deriv_dev(c, 1, 3, 0, c->s3, c->s6))) return rc;
+    HCHK(c, launch_ualpha_final(c->ualphas, c->s4, c->s5, c->s6, c->p.den0s, c->p.h2o2m4, c->p.alphas, c->pot4, n,
+                                c->st));
+    return 0;
+}
+
+// poten() (poten.f90:5-118): den -> pot4 (and dencg, delj4; OTC: denalf,
+// ualphas).  want_ccor / want_fden also store the correlation term in tmp
+// and the forward spectrum of den in Cs.
+static int poten_dev(hetd_ctx* c, bool want_ccor, bool want_fden) {
+    const long nl = (long)c->ny * c->nz;
+    XPassArgs<double> x{};
+    x.ax = c->ax[0]; x.nlines = nl; x.mode = XMODE_R2C; x.rin = c->den; x.cout = c->A;
+    HCHK(c, run_x(c, x));
+    YPassArgs<double> y{};
+    y.ax = c->ax[1]; y.nxh = c->nxh; y.inv = 0; y.buf[0] = c->A;
+    HCHK(c, run_y(c, y, 1));
+    // fden*W -> A, fden*V_LJ -> B [, fden*K_alpha -> Cs] (poten.f90:34-50, 96-101)
+    ZPassArgs<double> z{};
+    z.ax = c->ax[2]; z.nxh = c->nxh; z.ny = c->ny; z.mode = ZMODE_CONV; z.in[0] = c->A;
+    z.nk = c->otc ? 3 : 2;
+    z.out[0][0] = c->A; z.out[0][1] = c->B; z.out[0][2] = c->Cs;
+    z.kern[0] = c->wcgk; z.kern[1] = c->fvlj; z.kern[2] = c->kalfs;
+    z.fwd_out = want_fden ? c->Cs : nullptr;
+    if (want_fden && c->otc) return fail(c, HETD_ERR_STATE, "fden copy not available with OTC");
+    HCHK(c, run_z(c, z, 1));
+    y.inv = 1; y.buf[0] = c->A; y.buf[1] = c->B; y.buf[2] = c->Cs;
+    HCHK(c, run_y(c, y, c->otc ? 3 : 2));
+    // delj4 = c2r(fden*V)*renor
+    XPassArgs<double> xl{};
+    xl.ax = c->ax[0]; xl.nlines = nl; xl.mode = XMODE_C2R; xl.cin = c->B; xl.rout = c->delj4; xl.scale = c->renor;
+    HCHK(c, run_x(c, xl));
+    // dencg = c2r(fden*W)*renor, s = den*dencg*(cp4+cpp4*dencg), A = x-r2c(s)
+    XPassArgs<double> xc{};
+    xc.ax = c->ax[0]; xc.nlines = nl; xc.mode = XMODE_CG_S; xc.cin = c->A; xc.rin = c->den; xc.rout = c->dencg;
+    xc.cout = c->A; xc.scale = c->renor; xc.cp4 = c->p.cp4; xc.cpp4 = c->p.cpp4;
+    HCHK(c, run_x(c, xc));
+    y.inv = 0; y.buf[0] = c->A;
+    HCHK(c, run_y(c, y, 1));
+    ZPassArgs<double> z2{};
+    z2.ax = c->ax[2]; z2.nxh = c->nxh; z2.ny = c->ny; z2.mode = ZMODE_CONV; z2.nk = 1; z2.in[0] = c->A;
+    z2.out[0][0] = c->A; z2.kern[0] = c->wcgk;
+    HCHK(c, run_z(c, z2, 1));
+    y.inv = 1;
+    HCHK(c, run_y(c, y, 1));
+    // pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + sto1 [+ penalty] (poten.f90:80-94, 110-114)
+    XPassArgs<double> xp{};
+    xp.ax = c->ax[0]; xp.nlines = nl; xp.mode = XMODE_POT; xp.cin = c->A; xp.rin = c->den; xp.rin2 = c->dencg;
+    xp.rin3 = c->delj4; xp.rout = c->pot4; xp.rout2 = want_ccor ? c->tmp : nullptr; xp.scale = c->renor;
+    xp.a0 = c->p.cp4 / 2.0; xp.a1 = c->p.cpp4 / 3.0;
+    xp.lsolid = c->p.lsolid && !c->otc;
+    xp.csol = c->p.c_solid; xp.beta = c->p.beta_solid; xp.den_m = c->p.den_m;
+    HCHK(c, run_x(c, xp));
+    if (c->otc) {
+        XPassArgs<double> xa{};
+        xa.ax = c->ax[0]; xa.nlines = nl; xa.mode = XMODE_C2R; xa.cin = c->Cs; xa.rout = c->denalf;
+        xa.scale = c->renor;
+        HCHK(c, run_x(c, xa));
+        int rc = term_alfa_dev(c);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+// forceimp + aimp = F/mAg_u on the device (potenimp.f90:24-55, DFT4He3d.f90:852)
+static int force_dev(hetd_ctx* c) {
+    ForceArgs f{};
+    f.g = c->grid; f.sc = c->sc; f.den = c->den; f.imp = c->imp; f.partial = c->partial;
+    int nb = 0;
+    HCHK(c, run_force(c, f, &nb));
+    HCHK(c, launch_force_final(c->imp, c->partial, nb, c->dxyz, c->mAg, c->st));
+    return 0;
+}
+
+static int potential_update(hetd_ctx* c) {
+    int rc = poten_dev(c, false, false);
+    if (rc) return rc;
+    return force_dev(c);
+}
+
+static UpdateArgs update_base(hetd_ctx* c, double dt) {
+    UpdateArgs u{};
+    u.g = c->grid; u.sc = c->sc; u.h2o2m4 = c->p.h2o2m4; u.dt = dt; u.psi = c->psi; u.pot4 = c->pot4;
+    u.imp = c->imp; u.den = c->den; u.guard = c->guard; u.partial = c->partial;
+    return u;
+}
+
+static int steprk_dev(hetd_ctx* c, double dt, bool last_sets_rpot) {
+    const double s2 = std::sqrt(2.0);
+    const double ar[4] = {0.5, 1.0 - 1.0 / s2, 1.0 + 1.0 / s2, 1.0 / 6.0};
+    const double br[4] = {2.0, 1.0, 1.0, 2.0};
+    const double cr[4] = {0.5, 1.0 - 1.0 / s2, 1.0 + 1.0 / s2, 0.5};
+    for (int j = 0; j < 4; ++j) {
+        UpdateArgs u = update_base(c, dt);
+        u.kind = STAGE_RK; u.rk_j = j; u.ra = ar[j]; u.rb = br[j]; u.rc = cr[j];
+        u.q = c->q; u.psi_out = c->spare;
+        u.hnew = (j == 0) ? c->hpsiold[1] : nullptr;
+        HCHK(c, run_update(c, u, nullptr));
+        if (j == 0) {
+            std::swap(c->hpsiold[0], c->hpsiold[1]);
+            double2* t = c->psiold[2];
+            c->psiold[2] = c->psiold[1];
+            c->psiold[1] = c->psiold[0];
+            c->psiold[0] = c->psi;
+            c->psi = c->spare;
+            c->spare = t;
+        } else {
+            std::swap(c->psi, c->spare);
+        }
+        ImpRkArgs ia{c->imp, j, dt, ar[j], br[j], cr[j], (j <= 2 || last_sets_rpot) ? 1 : 0};
+        HCHK(c, launch_imp_rk(ia, c->st));
+        if (j <= 2) {
+            int rc = potential_update(c);
+            if (rc) return rc;
+        }
+    }
+    for (int i = 0; i < 3; ++i) c->ioldp[i] = c->ioldr[i] = c->ioldv[i] = i + 1;
+    for (int i = 0; i < 2; ++i) c->ioldh[i] = c->iolda[i] = i + 1;
+    return 0;
+}
+
+static int steppc_dev(hetd_ctx* c, double dt, bool sets_rpot) {
+    const int p3 = c->ioldp[2] - 1, p2 = c->ioldp[1] - 1;
+    {
+        UpdateArgs u = update_base(c, dt);
+        u.kind = STAGE_PRED;
+        u.po3 = c->psiold[p3]; u.psi_out = c->psiold[p3];
+        u.h1 = c->hpsiold[c->ioldh[0] - 1]; u.h2 = c->hpsiold[c->ioldh[1] - 1]; u.hnew = c->hpsiold[c->ioldh[1] - 1];
+        u.pc = c->pc;
+        HCHK(c, run_update(c, u, nullptr));
+        std::swap(c->psi, c->psiold[p3]);
+        std::swap(c->ioldh[0], c->ioldh[1]);
+    }
+    ImpPcArgs ia{c->imp, dt, c->ioldr[0] - 1, c->ioldr[1] - 1, c->ioldr[2] - 1, c->ioldv[0] - 1, c->ioldv[1] - 1,
+                 c->ioldv[2] - 1, c->iolda[0] - 1, c->iolda[1] - 1, 0};
+    HCHK(c, launch_imp_pred(ia, c->st));
+    std::swap(c->iolda[0], c->iolda[1]);
+    int rc = potential_update(c);
+    if (rc) return rc;
+    {
+        UpdateArgs u = update_base(c, dt);
+        u.kind = STAGE_CORR;
+        u.po3 = c->psiold[p3]; u.po2 = c->psiold[p2];
+        u.h1 = c->hpsiold[c->ioldh[0] - 1]; u.h2 = c->hpsiold[c->ioldh[1] - 1];
+        u.pc = c->pc; u.psi_out = c->spare;
+        int nb = 0;
+        HCHK(c, run_update(c, u, &nb));
+        HCHK(c, launch_errhe_final(c->imp, c->partial, nb, (double)c->n, c->st));
+        std::swap(c->psi, c->spare);
+        const int t = c->ioldp[2];
+        c->ioldp[2] = c->ioldp[1];
+        c->ioldp[1] = c->ioldp[0];
+        c->ioldp[0] = t;
+    }
+    ImpPcArgs ic{c->imp, dt, c->ioldr[0] - 1, c->ioldr[1] - 1, c->ioldr[2] - 1, c->ioldv[0] - 1, c->ioldv[1] - 1,
+                 c->ioldv[2] - 1, c->iolda[0] - 1, c->iolda[1] - 1, sets_rpot ? 1 : 0};
+    HCHK(c, launch_imp_corr(ic, c->st));
+    int t = c->ioldr[2]; c->ioldr[2] = c->ioldr[1]; c->ioldr[1] = c->ioldr[0]; c->ioldr[0] = t;
+    t = c->ioldv[2]; c->ioldv[2] = c->ioldv[1]; c->ioldv[1] = c->ioldv[0]; c->ioldv[0] = t;
+    return 0;
+}
+
+static int check_guard(hetd_ctx* c) {
+    int g = 0;
+    HCHK(c, hipMemcpyAsync(&g, c->guard, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    if (g) return fail(c, HETD_ERR_GUARD, "den not > 0 after a Runge-Kutta stage (steprk.f90:89-92)");
+    return 0;
+}
+
+// --------------------------------------------------------------------- ABI
+
+extern "C" {
+
+void hetd_default_params(hetd_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->nx = p->ny = p->nz = 64;
+    p->npd = 13;
+    p->icon = 13;
+    p->core = HETD_CORE_OT;
+    p->lsolid = 1;
+    p->precision = HETD_FP64;
+    p->xmax = p->ymax = p->zmax = 64.0;
+    p->eps4 = 10.22; p->sigma4 = 2.556; p->h4 = 2.190323;
+    p->cp4 = -2.41186e4; p->cpp4 = 1.85850e6; p->h2o2m4 = 6.05969638298;
+    p->alphas = 54.31; p->l_alpha = 1.0; p->den0s = 0.04;
+    p->c_solid = 3.1577504e4; p->beta_solid = 40.0; p->den_m = 0.37;
+    p->mimpur = 131.29;
+    p->Lambdah = 2.0;
+    p->txmean = p->tymean = p->tzmean = 1000.0;
+    p->txsurf = p->tysurf = p->tzsurf = 2.0;
+}
+
+const char* hetd_last_error(hetd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void* hetd_stream(hetd_ctx* c) { return c ? (void*)c->st : nullptr; }
+
+int hetd_vimp(const char* selec, double r, double r_cutoff, double umax, double* v) {
+    if (!selec || !v) return HETD_ERR_ARG;
+    std::string s(selec);
+    while (!s.empty() && s.back() == ' ') s.pop_back();
+    return setup::select_pot(s, r, r_cutoff, umax, v) ? HETD_OK : HETD_ERR_ARG;
+}
+
+int hetd_build_potion(const char* selec, double r_cutoff, double umax, int32_t nmax, double xmaxbox, double* potion,
+                      int32_t* npot, double* delinter) {
+    if (!selec || !npot || !delinter || nmax < 1) return HETD_ERR_ARG;
+    const int np = 100 * nmax + 1;
+    const double rmax = 4.0 * xmaxbox;
+    const double d = rmax / (double)(np - 1);
+    *npot = np;
+    *delinter = d;
+    if (!potion) return HETD_OK;
+    for (int i = 0; i < np; ++i) {
+        const double r = (double)i * d;
+        if (hetd_vimp(selec, r, r_cutoff, umax, &potion[i])) return HETD_ERR_ARG;
+    }
+    return HETD_OK;
+}
+
+int hetd_init(const hetd_params* p, hetd_ctx** out) {
+    if (!p || !out) return HETD_ERR_ARG;
+    *out = nullptr;
+    hetd_ctx* c = new hetd_ctx();
+    c->p = *p;
+    auto bad = [&](const char* m) { c->err = m; return HETD_ERR_ARG; };
+    int rc = 0;
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1) rc = bad("grid sizes must be positive");
+    else if (p->npd < 3 || p->npd > 25 || p->npd % 2 == 0) rc = bad("npd must be odd, 3..25 (newder.f:75-76)");
+    else if (p->icon != 13 && p->icon != 8 && p->icon != 18) rc = bad("icon must be 13, 8 or 18");
+    else if (p->precision != HETD_FP64) rc = bad("only HETD_FP64 is available in this build");
+    else if (p->nx > 1024 || p->ny > 1024 || p->nz > 1024) rc = bad("grid edge above 1024");
+    if (rc) { *out = c; return rc; }
+    c->nx = p->nx; c->ny = p->ny; c->nz = p->nz; c->nxh = p->nx / 2 + 1;
+    c->n = (long)c->nx * c->ny * c->nz;
+    c->nh = (long)c->nxh * c->ny * c->nz;
+    c->hx = 2.0 * std::fabs(p->xmax) / p->nx;
+    c->hy = 2.0 * std::fabs(p->ymax) / p->ny;
+    c->hz = 2.0 * std::fabs(p->zmax) / p->nz;
+    c->dxyz = c->hx * c->hy * c->hz;
+    c->mAg = p->mimpur * 0.02061484;  // mp_u (modules.f90:315)
+    c->renor = 1.0 / (double)(c->nx * c->ny * c->nz);  // fft.f90:36
+    c->otc = (!p->lsolid && p->core == HETD_CORE_OTC);
+    *out = c;
+    const int nn[3] = {c->nx, c->ny, c->nz};
+    for (int a = 0; a < 3; ++a) {
+        if (!setup::radix_plan(nn[a], c->ax[a].rad, &c->ax[a].nst, kMaxStages))
+            return fail(c, HETD_ERR_ARG, "grid edge %d has a prime factor above 5", nn[a]);
+        c->ax[a].n = nn[a];
+    }
+    HCHK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    // grid tables (DFT4He3d.f90:309-317, 689-699)
+    const double hh[3] = {c->hx, c->hy, c->hz}, mx[3] = {p->xmax, p->ymax, p->zmax};
+    const double tmean[3] = {p->txmean, p->tymean, p->tzmean}, tsurf[3] = {p->txsurf, p->tysurf, p->tzsurf};
+    double** dco[3] = {&c->d_x, &c->d_y, &c->d_z};
+    double** dtc[3] = {&c->d_tcx, &c->d_tcy, &c->d_tcz};
+    for (int a = 0; a < 3; ++a) {
+        std::vector<double> co(nn[a]), tc(nn[a]);
+        for (int i = 0; i < nn[a]; ++i) {
+            co[i] = -mx[a] + hh[a] * i;
+            tc[i] = std::tanh((std::fabs(co[i]) - tmean[a]) / tsurf[a]);
+        }
+        HCHK(c, dalloc(c, dco[a], sizeof(double) * nn[a]));
+        HCHK(c, dalloc(c, dtc[a], sizeof(double) * nn[a]));
+        HCHK(c, hipMemcpy(*dco[a], co.data(), sizeof(double) * nn[a], hipMemcpyHostToDevice));
+        HCHK(c, hipMemcpy(*dtc[a], tc.data(), sizeof(double) * nn[a], hipMemcpyHostToDevice));
+        std::vector<double> t64;
+        std::vector<float> t32;
+        setup::twiddles(nn[a], t64, t32);
+        HCHK(c, dalloc(c, &c->d_tw[a], sizeof(double) * t64.size()));
+        HCHK(c, dalloc(c, &c->d_tw32[a], sizeof(float) * t32.size()));
+        HCHK(c, hipMemcpy(c->d_tw[a], t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice));
+        HCHK(c, hipMemcpy(c->d_tw32[a], t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice));
+        c->ax[a].tw64 = c->d_tw[a];
+        c->ax[a].tw32 = c->d_tw32[a];
+    }
+    // stencil weights * 1/h**id (init_deriv_parallel.f90:415-431)
+    c->sc.npd = p->npd; c->sc.m = p->npd / 2; c->sc.icon = p->icon;
+    double w1[kMaxNpd], w2[kMaxNpd];
+    setup::fd_weights(p->npd, 1, w1);
+    setup::fd_weights(p->npd, 2, w2);
+    for (int a = 0; a < 3; ++a) {
+        const double s1 = 1.0 / hh[a], s2 = 1.0 / (hh[a] * hh[a]);
+        for (int j = 0; j < p->npd; ++j) {
+            c->sc.c1[a][j] = w1[j] * s1;
+            c->sc.c2[a][j] = w2[j] * s2;
+        }
+    }
+    // fields
+    const size_t nR = sizeof(double) * c->n, nC = sizeof(double2) * c->n, nH = sizeof(double2) * c->nh;
+    HCHK(c, dalloc(c, &c->psi, nC));
+    for (int i = 0; i < 3; ++i) HCHK(c, dalloc(c, &c->psiold[i], nC));
+    HCHK(c, dalloc(c, &c->spare, nC));
+    for (int i = 0; i < 2; ++i) HCHK(c, dalloc(c, &c->hpsiold[i], nC));
+    HCHK(c, dalloc(c, &c->pc, nC));
+    HCHK(c, dalloc(c, &c->q, nC));
+    HCHK(c, dalloc(c, &c->den, nR));
+    HCHK(c, dalloc(c, &c->pot4, nR));
+    HCHK(c, dalloc(c, &c->dencg, nR));
+    HCHK(c, dalloc(c, &c->delj4, nR));
+    HCHK(c, dalloc(c, &c->tmp, nR));
+    HCHK(c, dalloc(c, &c->A, nH));
+    HCHK(c, dalloc(c, &c->B, nH));
+    HCHK(c, dalloc(c, &c->Cs, nH));
+    HCHK(c, dalloc(c, &c->fvlj, sizeof(double) * c->nh));
+    HCHK(c, dalloc(c, &c->wcgk, sizeof(double) * c->nh));
+    HCHK(c, dalloc(c, &c->kalfs, sizeof(double) * c->nh));
+    if (c->otc) {
+        double** arr[] = {&c->dxden, &c->dyden, &c->dzden, &c->denalf, &c->falfs, &c->intx, &c->inty, &c->intz,
+                          &c->ualphas, &c->s1, &c->s2, &c->s3, &c->s4, &c->s5, &c->s6};
+        for (auto a : arr) HCHK(c, dalloc(c, a, nR));
+    }
+    HCHK(c, dalloc(c, &c->imp, sizeof(ImpState)));
+    HCHK(c, dalloc(c, &c->guard, sizeof(int)));
+    const long nblocks = (long)((c->nx + 63) / 64) * ((c->ny + 3) / 4) * c->nz;
+    HCHK(c, dalloc(c, &c->partial, sizeof(double) * nblocks * EN_NCOMP));
+    HCHK(c, dalloc(c, &c->red, sizeof(double) * 16));
+    HCHK(c, hipMemset(c->imp, 0, sizeof(ImpState)));
+    HCHK(c, hipMemset(c->guard, 0, sizeof(int)));
+    // form factors on the reference p-grid (fforma.f90, initcg.f90, DFT4He3d.f90:592-596)
+    {
+        std::vector<double> fv(c->nh), wc(c->nh), ka(c->nh);
+        const double hcg = p->lsolid ? p->h4 * 1.065 : p->h4;
+        const double pi = 4.0 * std::atan(1.0);
+        const int nxh = c->nxh, ny = c->ny, nz = c->nz;
+        const double hx = c->hx, hy = c->hy, hz = c->hz;
+        setup::parallel_for(nz, [&](int k) {
+            const double pz = setup::pgrid(k, nz, hz);
+            for (int j = 0; j < ny; ++j) {
+                const double py = setup::pgrid(j, ny, hy);
+                for (int i = 0; i < nxh; ++i) {
+                    const double px = (1.0 / (c->nx * hx)) * i;  // hpx*(ipx-1), ipx <= nx/2+1
+                    const double pm = std::sqrt(px * px + py * py + pz * pz);
+                    const size_t o = (size_t)i + (size_t)nxh * ((size_t)j + (size_t)ny * k);
+                    fv[o] = setup::vlj_hat(pm, p->eps4, p->sigma4, p->h4);
+                    wc[o] = setup::wcg(pm, hcg);
+                    ka[o] = std::exp(-(pi * p->l_alpha * pm) * (pi * p->l_alpha * pm));
+                }
+            }
+        });
+        HCHK(c, hipMemcpy(c->fvlj, fv.data(), sizeof(double) * c->nh, hipMemcpyHostToDevice));
+        HCHK(c, hipMemcpy(c->wcgk, wc.data(), sizeof(double) * c->nh, hipMemcpyHostToDevice));
+        HCHK(c, hipMemcpy(c->kalfs, ka.data(), sizeof(double) * c->nh, hipMemcpyHostToDevice));
+    }
+    fill_grid(c);
+    return HETD_OK;
+}
+
+int hetd_finalize(hetd_ctx* c) {
+    if (!c) return HETD_ERR_ARG;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (void* v : c->allocs) (void)hipFree(v);
+    if (c->traj) (void)hipFree(c->traj);
+    if (c->dtmp[0]) (void)hipFree(c->dtmp[0]);
+    if (c->dtmp[1]) (void)hipFree(c->dtmp[1]);
+    for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return HETD_OK;
+}
+
+int hetd_set_potion(hetd_ctx* c, const double* potion, int32_t npot, double delinter) {
+    if (!c || !potion || npot < 2 || !(delinter > 0)) return HETD_ERR_ARG;
+    if (c->potion && c->npot != npot) return fail(c, HETD_ERR_ARG, "table size changed");
+    if (!c->potion) HCHK(c, dalloc(c, &c->potion, sizeof(double) * npot));
+    HCHK(c, hipMemcpyAsync(c->potion, potion, sizeof(double) * npot, hipMemcpyHostToDevice, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    c->npot = npot;
+    c->delinter = delinter;
+    fill_grid(c);
+    return HETD_OK;
+}
+
+int hetd_set_state(hetd_ctx* c, const double* psi, const double r[3], const double v[3]) {
+    if (!c || !psi || !r || !v) return HETD_ERR_ARG;
+    if (!c->potion) return fail(c, HETD_ERR_STATE, "hetd_set_potion must precede hetd_set_state");
+    const size_t nC = sizeof(double2) * c->n;
+    HCHK(c, hipMemcpyAsync(c->psi, psi, nC, hipMemcpyHostToDevice, c->st));
+    HCHK(c, launch_den_from_psi(c->psi, c->den, c->n, c->st));
+    HCHK(c, hipMemsetAsync(c->q, 0, nC, c->st));
+    HCHK(c, hipMemsetAsync(c->pc, 0, nC, c->st));
+    for (int i = 0; i < 3; ++i) HCHK(c, hipMemsetAsync(c->psiold[i], 0, nC, c->st));
+    for (int i = 0; i < 2; ++i) HCHK(c, hipMemsetAsync(c->hpsiold[i], 0, nC, c->st));
+    ImpState s;
+    std::memset(&s, 0, sizeof s);
+    for (int d = 0; d < 3; ++d) { s.r[d] = r[d]; s.v[d] = v[d]; s.rpot[d] = r[d]; }
+    HCHK(c, hipMemcpyAsync(c->imp, &s, sizeof s, hipMemcpyHostToDevice, c->st));
+    HCHK(c, hipMemsetAsync(c->guard, 0, sizeof(int), c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    for (int i = 0; i < 3; ++i) c->ioldp[i] = c->ioldr[i] = c->ioldv[i] = i + 1;
+    for (int i = 0; i < 2; ++i) c->ioldh[i] = c->iolda[i] = i + 1;
+    c->iter_since_set = 0;
+    c->have_state = true;
+    return HETD_OK;
+}
+
+int hetd_potenimp(hetd_ctx* c, const double r[3]) {
+    if (!c || !r) return HETD_ERR_ARG;
+    HCHK(c, hipMemcpyAsync(c->imp->rpot, r, 3 * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    return HETD_OK;
+}
+
+int hetd_poten(hetd_ctx* c) {
+    if (!c) return HETD_ERR_ARG;
+    if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
+    int rc = poten_dev(c, false, false);
+    if (rc) return rc;
+    HCHK(c, hipStreamSynchronize(c->st));
+    return HETD_OK;
+}
+
+int hetd_forceimp(hetd_ctx* c, const double r[3], double F[3]) {
+    (void)r;  // the reference's forceimp uses the u_imp of the last potenimp
+    if (!c || !F) return HETD_ERR_ARG;
+    if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
+    int rc = force_dev(c);
+    if (rc) return rc;
+    HCHK(c, hipMemcpyAsync(F, c->imp->F, 3 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    return HETD_OK;
+}
+
+int hetd_set_aimp(hetd_ctx* c, const double a[3]) {
+    if (!c || !a) return HETD_ERR_ARG;
+    HCHK(c, hipMemcpyAsync(c->imp->a, a, 3 * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    return HETD_OK;
+}
+
+int hetd_steprk(hetd_ctx* c, double dt) {
+    if (!c) return HETD_ERR_ARG;
+    if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
+    int rc = steprk_dev(c, dt, false);
+    if (rc) return rc;
+    c->iter_since_set++;
+    return check_guard(c);
+}
+
+int hetd_steppc(hetd_ctx* c, double dt, double* errHe, double* errimp) {
+    if (!c || !errHe || !errimp) return HETD_ERR_ARG;
+    if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
+    int rc = steppc_dev(c, dt, false);
+    if (rc) return rc;
+    c->iter_since_set++;
+    double e[2];
+    HCHK(c, hipMemcpyAsync(e, &c->imp->errHe, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    *errHe = e[0];
+    *errimp = e[1];
+    return HETD_OK;
+}
+
+int hetd_evolve(hetd_ctx* c, double dt, int32_t nsteps, double* traj) {
+    if (!c || nsteps < 0) return HETD_ERR_ARG;
+    if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
+    if (traj && c->traj_cap < (long)nsteps * HETD_TRAJ_STRIDE) {
+        if (c->traj) {
+            HCHK(c, hipStreamSynchronize(c->st));
+            HCHK(c, hipFree(c->traj));
+            c->traj = nullptr;
+        }
+        HCHK(c, hipMalloc(&c->traj, sizeof(double) * (size_t)nsteps * HETD_TRAJ_STRIDE));
+        c->traj_cap = (long)nsteps * HETD_TRAJ_STRIDE;
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        int rc;
+        c->iter_since_set++;
+        if (c->iter_since_set <= 3) {
+            // errHe/errimp are zero for RK steps (the driver prints them only after PC)
+            HCHK(c, hipMemsetAsync(&c->imp->errHe, 0, 2 * sizeof(double), c->st));
+            rc = steprk_dev(c, dt, true);
+        } else {
+            rc = steppc_dev(c, dt, true);
+        }
+        if (rc) return rc;
+        // driver: potenimp(rimp); poten(); forceimp(rimp,aimp); aimp/=m (DFT4He3d.f90:849-852)
+        rc = potential_update(c);
+        if (rc) return rc;
+        if (traj) HCHK(c, launch_traj_record(c->imp, c->traj + (long)s * HETD_TRAJ_STRIDE, c->st));
+    }
+    if (traj && nsteps > 0)
+        HCHK(c, hipMemcpyAsync(traj, c->traj, sizeof(double) * (size_t)nsteps * HETD_TRAJ_STRIDE,
+                               hipMemcpyDeviceToHost, c->st));
+    return check_guard(c);
+}
+
+int hetd_energy(hetd_ctx* c, hetd_energies* e) {
+    if (!c || !e) return HETD_ERR_ARG;
+    if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
+    EnergyArgs a{};
+    a.g = c->grid; a.sc = c->sc; a.psi = c->psi; a.den = c->den; a.dencg = c->dencg; a.delj4 = c->delj4;
+    a.imp = c->imp; a.aux1 = 0.5 * c->p.cp4; a.aux2 = (1.0 / 3.0) * c->p.cpp4; a.beta = c->p.beta_solid;
+    a.den_m = c->p.den_m; a.lsolid = c->p.lsolid; a.otc = c->otc ? 1 : 0;
+    a.falfs = c->falfs; a.dxden = c->dxden; a.dyden = c->dyden; a.dzden = c->dzden;
+    a.intx = c->intx; a.inty = c->inty; a.intz = c->intz; a.partial = c->partial;
+    int nb = 0;
+    HCHK(c, launch_energy_partial(a, c->st, &nb));
+    HCHK(c, launch_reduce_final(c->partial, nb, EN_NCOMP, c->red, c->st));
+    double s[EN_NCOMP];
+    ImpState is;
+    HCHK(c, hipMemcpyAsync(s, c->red, sizeof s, hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipMemcpyAsync(&is, c->imp, sizeof is, hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    const hetd_params& p = c->p;
+    // energy.f90:99-132
+    e->ealphas = c->otc ? -p.h2o2m4 * 0.5 * p.alphas * s[EN_ALPHA] * c->dxyz : 0.0;
+    e->esolid = p.lsolid ? p.c_solid * s[EN_SOLID] * c->dxyz : 0.0;
+    e->ekin4 = p.h2o2m4 * s[EN_KIN] * c->dxyz;
+    e->elj4 = 0.5 * s[EN_LJ] * c->dxyz;
+    e->ecor4 = s[EN_COR] * c->dxyz;
+    e->etot4 = e->ekin4 + e->elj4 + e->ecor4 + e->esolid;
+    if (c->otc) e->etot4 = e->etot4 + e->ealphas;
+    e->ekinx = 0.5 * c->mAg * (is.v[0] * is.v[0] + is.v[1] * is.v[1] + is.v[2] * is.v[2]);
+    e->eHeX = s[EN_HEX] * c->dxyz;
+    e->eimpu = e->ekinx + e->eHeX;
+    e->etot = e->etot4 + e->eimpu;
+    return HETD_OK;
+}
+
+int hetd_norm(hetd_ctx* c, double* norm) {
+    if (!c || !norm) return HETD_ERR_ARG;
+    EnergyArgs a{};
+    a.g = c->grid; a.sc = c->sc; a.psi = c->psi; a.den = c->den; a.dencg = c->dencg; a.delj4 = c->delj4;
+    a.imp = c->imp; a.partial = c->partial;
+    int nb = 0;
+    HCHK(c, launch_energy_partial(a, c->st, &nb));
+    HCHK(c, launch_reduce_final(c->partial, nb, EN_NCOMP, c->red, c->st));
+    double s[EN_NCOMP];
+    HCHK(c, hipMemcpyAsync(s, c->red, sizeof s, hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    *norm = s[EN_NORM] * c->dxyz;
+    return HETD_OK;
+}
+
+int hetd_get_imp(hetd_ctx* c, double r[3], double v[3], double a[3]) {
+    if (!c || !r || !v || !a) return HETD_ERR_ARG;
+    ImpState is;
+    HCHK(c, hipMemcpyAsync(&is, c->imp, sizeof is, hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    for (int d = 0; d < 3; ++d) { r[d] = is.r[d]; v[d] = is.v[d]; a[d] = is.a[d]; }
+    return HETD_OK;
+}
+
+int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
+    if (!c || !host) return HETD_ERR_ARG;
+    const size_t nR = sizeof(double) * c->n, nC = sizeof(double2) * c->n, nHr = sizeof(double) * c->nh;
+    const void* src = nullptr;
+    size_t bytes = nR;
+    int rc = 0;
+    switch (which) {
+    case HETD_F_PSI: src = c->psi; bytes = nC; break;
+    case HETD_F_DEN: src = c->den; break;
+    case HETD_F_POT4: src = c->pot4; break;
+    case HETD_F_DENCG: src = c->dencg; break;
+    case HETD_F_DELJ4: src = c->delj4; break;
+    case HETD_F_PC: src = c->pc; bytes = nC; break;
+    case HETD_F_Q: src = c->q; bytes = nC; break;
+    case HETD_F_FVLJ: src = c->fvlj; bytes = nHr; break;
+    case HETD_F_WCGK: src = c->wcgk; bytes = nHr; break;
+    case HETD_F_CCOR:
+    case HETD_F_FDEN: {
+        // recompute the last poten (den is unchanged since) keeping the
+        // intermediate; pot4/dencg/delj4 are reproduced bit for bit
+        if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
+        if (which == HETD_F_FDEN && c->otc) return fail(c, HETD_ERR_ARG, "FDEN not kept with OTC");
+        if ((rc = poten_dev(c, which == HETD_F_CCOR, which == HETD_F_FDEN))) return rc;
+        src = which == HETD_F_CCOR ? (const void*)c->tmp : (const void*)c->Cs;
+        bytes = which == HETD_F_CCOR ? nR : sizeof(double2) * c->nh;
+        break;
+    }
+    case HETD_F_DXDEN:
+    case HETD_F_DYDEN:
+    case HETD_F_DZDEN:
+        if ((rc = deriv_dev(c, 1, which - HETD_F_DXDEN + 1, 0, c->den, c->tmp))) return rc;
+        src = c->tmp;
+        break;
+    case HETD_F_UIMP:
+        HCHK(c, launch_uimp_field(c->grid, c->imp, c->tmp, c->st));
+        src = c->tmp;
+        break;
+    case HETD_F_PENALTY: {
+        if (!c->p.lsolid) return fail(c, HETD_ERR_ARG, "no penalty without lsolid");
+        std::vector<double> d(c->n);
+        HCHK(c, hipMemcpyAsync(d.data(), c->den, nR, hipMemcpyDeviceToHost, c->st));
+        HCHK(c, hipStreamSynchronize(c->st));
+        for (long i = 0; i < c->n; ++i) {
+            const double t = std::tanh(c->p.beta_solid * (d[i] - c->p.den_m));
+            host[i] = c->p.c_solid * (1.0 + t + c->p.beta_solid * d[i] * (1.0 - t * t));
+        }
+        return HETD_OK;
+    }
+    case HETD_F_UALPHAS:
+    case HETD_F_DENALF:
+        if (!c->otc) return fail(c, HETD_ERR_ARG, "alpha_s fields exist only with core OTC");
+        src = which == HETD_F_UALPHAS ? c->ualphas : c->denalf;
+        break;
+    default:
+        return fail(c, HETD_ERR_ARG, "unknown field %d", which);
+    }
+    HCHK(c, hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    return HETD_OK;
+}
+
+int hetd_fft_r2c(hetd_ctx* c, const double* in, double* out) {
+    if (!c || !in || !out) return HETD_ERR_ARG;
+    HCHK(c, hipMemcpyAsync(c->tmp, in, sizeof(double) * c->n, hipMemcpyHostToDevice, c->st));
+    int rc = r2c_dev(c, c->tmp, c->A);
+    if (rc) return rc;
+    HCHK(c, hipMemcpyAsync(out, c->A, sizeof(double2) * c->nh, hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    return HETD_OK;
+}
+
+int hetd_fft_c2r(hetd_ctx* c, const double* in, double* out) {
+    if (!c || !in || !out) return HETD_ERR_ARG;
+    HCHK(c, hipMemcpyAsync(c->A, in, sizeof(double2) * c->nh, hipMemcpyHostToDevice, c->st));
+    int rc = c2r_dev(c, c->A, c->tmp, c->renor);
+    if (rc) return rc;
+    HCHK(c, hipMemcpyAsync(out, c->tmp, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    return HETD_OK;
+}
+
+int hetd_deriv(hetd_ctx* c, int32_t id, int32_t iv, int32_t is_complex, const double* f, double* df) {
+    if (!c || !f || !df || id < 1 || id > 2 || iv < 1 || iv > 3) return HETD_ERR_ARG;
+    const size_t bytes = (is_complex ? sizeof(double2) : sizeof(double)) * c->n;
+    for (int i = 0; i < 2; ++i)
+        if (!c->dtmp[i]) HCHK(c, hipMalloc(&c->dtmp[i], sizeof(double2) * c->n));
+    HCHK(c, hipMemcpyAsync(c->dtmp[0], f, bytes, hipMemcpyHostToDevice, c->st));
+    int rc = deriv_dev(c, id, iv, is_complex ? 1 : 0, c->dtmp[0], c->dtmp[1]);
+    if (rc) return rc;
+    HCHK(c, hipMemcpyAsync(df, c->dtmp[1], bytes, hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    return HETD_OK;
+}
+
+int hetd_timing(hetd_ctx* c, int32_t kclass) {
+    if (!c || kclass < HETD_K_OFF || kclass > HETD_K_FORCE) return HETD_ERR_ARG;
+    HCHK(c, hipStreamSynchronize(c->st));
+    c->tclass = kclass;
+    c->tused = 0;
+    c->tlaunch = 0;
+    c->tbytes = 0;
+    return HETD_OK;
+}
+
+int hetd_timing_read(hetd_ctx* c, double* total_ms, int64_t* launches, double* bytes) {
+    if (!c || !total_ms || !launches || !bytes) return HETD_ERR_ARG;
+    HCHK(c, hipStreamSynchronize(c->st));
+    double t = 0;
+    for (size_t i = 0; i + 1 < c->tused; i += 2) {
+        float ms = 0;
+        HCHK(c, hipEventElapsedTime(&ms, c->tev[i], c->tev[i + 1]));
+        t += ms;
+    }
+    *total_ms = t;
+    *launches = c->tlaunch;
+    *bytes = c->tbytes;
+    return HETD_OK;
+}
+
+}  // extern "C"

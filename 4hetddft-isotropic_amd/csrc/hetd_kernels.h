@@ -1,0 +1,178 @@
+// hetd_kernels.h -- kernel argument blocks and host-side launchers.
+#pragma once
+
+#include "hetd_dev.h"
+
+namespace hetd {
+
+// ---------------------------------------------------------------- FFT passes
+
+enum XMode { XMODE_R2C = 0, XMODE_C2R = 1, XMODE_CG_S = 2, XMODE_POT = 3 };
+enum ZMode { ZMODE_FWD = 0, ZMODE_INV = 1, ZMODE_CONV = 2 };
+
+template <typename T> struct XPassArgs {
+    FftAxis ax;
+    long nlines;  // ny*nz
+    int ppb;      // line pairs per block (set by the launcher)
+    int mode;
+    T scale;      // applied after an inverse transform (renor)
+    const T* rin;        // R2C: input; CG_S: den; POT: den
+    const T* rin2;       // POT: dencg
+    const T* rin3;       // POT: delj4
+    const cplx<T>* cin;  // inverse input half spectrum
+    T* rout;             // C2R output; CG_S: dencg; POT: pot4
+    T* rout2;            // POT: correlation term sto1 (optional)
+    cplx<T>* cout;       // forward output half spectrum
+    T cp4, cpp4, a0, a1;
+    int lsolid;
+    T csol, beta, den_m;
+};
+
+template <typename T> struct YPassArgs {
+    FftAxis ax;
+    int nxh, tx, inv;
+    cplx<T>* buf[4];
+};
+
+template <typename T> struct ZPassArgs {
+    FftAxis ax;
+    int nxh, ny, tx, mode, nk;
+    const cplx<T>* in[4];
+    cplx<T>* out[4][3];
+    const T* kern[3];
+    cplx<T>* fwd_out;  // optional copy of the forward spectrum (CONV)
+};
+
+template <typename T> hipError_t launch_xpass(XPassArgs<T> a, hipStream_t st);
+template <typename T> hipError_t launch_ypass(YPassArgs<T> a, int nz, int nbuf, hipStream_t st);
+template <typename T> hipError_t launch_zpass(ZPassArgs<T> a, int nbuf, hipStream_t st);
+
+// --------------------------------------------------------------- grid info
+
+// Stencil weights already divided by h**id (init_deriv_parallel.f90:415-431).
+struct StencilCoef {
+    int npd, m, icon;
+    double c1[3][kMaxNpd];  // first derivative per axis
+    double c2[3][kMaxNpd];  // second derivative per axis
+};
+
+struct Grid {
+    int nx, ny, nz;
+    long n;
+    double dxyz;
+    const double* x;  // coordinates x(i) = -xmax + hx*(i-1)
+    const double* y;
+    const double* z;
+    const double* tcx;  // tanh((|x|-txmean)/txsurf) per axis (DFT4He3d.f90:689-699)
+    const double* tcy;
+    const double* tcz;
+    double lambdah;
+    const double* potion;  // impurity table (potenimpini.f90)
+    int npot;
+    double delinter;
+};
+
+// -------------------------------------------------------- update kernels
+
+enum StageKind { STAGE_RK = 0, STAGE_PRED = 1, STAGE_CORR = 2 };
+
+struct UpdateArgs {
+    Grid g;
+    StencilCoef sc;
+    double h2o2m4;
+    int kind;
+    int rk_j;  // RK stage 0..3
+    double dt;
+    double ra, rb, rc;  // Gill coefficients of the stage
+    const double2* psi;  // stencil input
+    const double* pot4;
+    const ImpState* imp;
+    double2* psi_out;
+    double* den;
+    double2* q;       // RK
+    double2* hnew;    // RK j=0: f -> hpsiold(1); PRED: f -> hpsiold(ioldh(2))
+    const double2* po3;  // PRED / CORR: psiold(ioldp(3))
+    const double2* po2;  // CORR: psiold(ioldp(2))
+    const double2* h1;   // hpsiold(ioldh(1))
+    const double2* h2;   // hpsiold(ioldh(2))
+    double2* pc;
+    double* partial;  // CORR: per-block sum |pc|
+    int* guard;       // RK: set if den not > 0
+};
+
+hipError_t launch_update(const UpdateArgs& a, hipStream_t st, int* nblocks_out);
+
+// --------------------------------------------------------- derivatives
+
+struct DerivArgs {
+    int nx, ny, nz;
+    StencilCoef sc;
+    int id, iv, cplx;
+    const double* f;
+    double* df;
+};
+hipError_t launch_deriv(const DerivArgs& a, hipStream_t st);
+
+// ---------------------------------------------------------- reductions
+
+struct ForceArgs {
+    Grid g;
+    StencilCoef sc;
+    const double* den;
+    const ImpState* imp;
+    double* partial;  // 3 per block
+};
+hipError_t launch_force_partial(const ForceArgs& a, hipStream_t st, int* nblocks);
+
+enum { EN_KIN = 0, EN_LJ, EN_COR, EN_SOLID, EN_HEX, EN_ALPHA, EN_NORM, EN_NCOMP };
+struct EnergyArgs {
+    Grid g;
+    StencilCoef sc;
+    const double2* psi;
+    const double* den;
+    const double* dencg;
+    const double* delj4;
+    const ImpState* imp;
+    double aux1, aux2, beta, den_m;
+    int lsolid, otc;
+    const double* falfs;
+    const double* dxden;
+    const double* dyden;
+    const double* dzden;
+    const double* intx;
+    const double* inty;
+    const double* intz;
+    double* partial;  // EN_NCOMP per block
+};
+hipError_t launch_energy_partial(const EnergyArgs& a, hipStream_t st, int* nblocks);
+
+// out[c] = sum_b partial[b*ncomp + c], fixed order (bitwise reproducible)
+hipError_t launch_reduce_final(const double* partial, int nblocks, int ncomp, double* out, hipStream_t st);
+
+// ------------------------------------------------------------ impurity ODE
+
+struct ImpRkArgs { ImpState* imp; int j; double dt, a, b, c; int set_rpot; };
+struct ImpPcArgs { ImpState* imp; double dt; int r1, r2, r3, v1, v2, v3, a1, a2; int set_rpot; };
+hipError_t launch_imp_rk(const ImpRkArgs& a, hipStream_t st);
+hipError_t launch_imp_pred(const ImpPcArgs& a, hipStream_t st);   // sets rpot = predicted R
+hipError_t launch_imp_corr(const ImpPcArgs& a, hipStream_t st);   // set_rpot: rpot = corrected R
+// F = -sum*dxyz, a = F/mass; from reduced sums in imp->F (overwritten)
+hipError_t launch_force_final(ImpState* imp, const double* partial, int nblocks, double dxyz, double mass,
+                              hipStream_t st);
+hipError_t launch_errhe_final(ImpState* imp, const double* partial, int nblocks, double npoints, hipStream_t st);
+hipError_t launch_traj_record(const ImpState* imp, double* rec, hipStream_t st);
+
+// ---------------------------------------------------------- pointwise
+
+hipError_t launch_den_from_psi(const double2* psi, double* den, long n, hipStream_t st);
+hipError_t launch_uimp_field(const Grid& g, const ImpState* imp, double* out, hipStream_t st);
+// OTC helpers (term_alfa.f90)
+hipError_t launch_falfs(const double* denalf, double den0s, double* falfs, long n, hipStream_t st);
+hipError_t launch_mul(const double* a, const double* b, double* out, long n, hipStream_t st);
+hipError_t launch_dot3(const double* a1, const double* b1, const double* a2, const double* b2,
+                       const double* a3, const double* b3, double* out, long n, hipStream_t st);
+hipError_t launch_ualpha_final(double* ua, const double* s4, const double* s5, const double* s6,
+                               double den0s, double h2o2m4, double alphas, double* pot4, long n, hipStream_t st);
+hipError_t launch_scale(double* a, double s, long n, hipStream_t st);
+
+}  // namespace hetd

@@ -26,6 +26,8 @@ COMPLEX_FIELDS = (F_PSI, F_PC, F_Q)
 HALF_FIELDS = (F_FVLJ, F_WCGK)
 HALF_COMPLEX_FIELDS = (F_FDEN,)
 TRAJ_STRIDE = 11
+K_OFF, K_UPDATE, K_XPASS, K_YPASS, K_ZPASS, K_FORCE = range(6)
+K_NAMES = {K_UPDATE: "update", K_XPASS: "xpass", K_YPASS: "ypass", K_ZPASS: "zpass", K_FORCE: "force"}
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libhetd.so")
@@ -121,6 +123,10 @@ def _proto(lib):
     lib.hetd_deriv.argtypes = [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp]
     lib.hetd_stream.argtypes = [_vp]
     lib.hetd_stream.restype = _vp
+    lib.hetd_timing.argtypes = [_vp, C.c_int32]
+    lib.hetd_timing.restype = C.c_int
+    lib.hetd_timing_read.argtypes = [_vp, _dp, C.POINTER(C.c_int64), _dp]
+    lib.hetd_timing_read.restype = C.c_int
     for name in ("hetd_init", "hetd_finalize", "hetd_vimp", "hetd_build_potion",
                  "hetd_set_potion", "hetd_set_state", "hetd_potenimp", "hetd_poten",
                  "hetd_forceimp", "hetd_set_aimp", "hetd_steprk", "hetd_steppc",
@@ -135,7 +141,7 @@ EXPORTED = ("hetd_default_params", "hetd_init", "hetd_finalize", "hetd_last_erro
             "hetd_potenimp", "hetd_poten", "hetd_forceimp", "hetd_set_aimp",
             "hetd_steprk", "hetd_steppc", "hetd_energy", "hetd_norm", "hetd_evolve",
             "hetd_get_field", "hetd_get_imp", "hetd_fft_r2c", "hetd_fft_c2r",
-            "hetd_deriv", "hetd_stream")
+            "hetd_deriv", "hetd_stream", "hetd_timing", "hetd_timing_read")
 
 _LIB = None
 

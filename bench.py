@@ -1,0 +1,227 @@
+"""Throughput of the real-time propagation step on one MI355X.
+
+Workload: He-1000 droplet + Xe impurity on a 256^3 grid at 0.4 A, Orsay-Trento
+functional with the solid penalty (the reference default), 13-point
+stencils, zero halo (icon 13), dt = 0.5 fs.  A "step" is one steady-state
+iteration of the reference loop (DFT4He3d.f90:843-852): Hamming
+predictor-corrector with its middle mean field, then the driver's mean field
+at the corrected state.  Inputs are resident in HBM when the timed region
+starts; the first three (Runge-Kutta) iterations are part of the warm-up.
+
+Prints one JSON line (the benchmark contract) with the roofline of the
+dominant kernel class, measured with HIP events on the library's stream, and
+a CPU baseline timed on this host: the reference's own Fortran routines
+(oracle/_ref/ref_harness, flang + MKL FFTW3) when built, else the C oracle.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "4hetddft-isotropic_amd")
+sys.path.insert(0, PKG)
+
+KTOPS = 7.63822291
+HBM_PEAK = 8.0e12  # B/s, MI355X
+METRIC = "Mpoint-steps/s, He-1000+imp 256^3: mixed vs FP64 GPU at ≤1e-6 ΔE/E; % HBM roof"
+
+
+def model_bytes_per_point_step(nx, mixed=False):
+    """Algorithmic HBM bytes per grid point per steady-state PC step:
+    448 + 8 s_h (FP64) or 360 + 4 s_h (mixed), s_h = 16 (nx/2+1)/nx."""
+    sh = 16.0 * (nx // 2 + 1) / nx
+    return 360.0 + 4.0 * sh if mixed else 448.0 + 8.0 * sh
+
+
+class HipEvents:
+    def __init__(self):
+        self.L = C.CDLL("libamdhip64.so")
+        self.L.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.L.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.L.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.L.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.L.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.L.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.L.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def cpu_baseline(n, nthreads, steps=2):
+    """Reference CPU path on this host's cores, a bounded sample: `steps`
+    predictor-corrector iterations of the same 256^3 workload."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if os.path.exists(harness):
+        work = tempfile.mkdtemp(prefix="hetd_cpu_")
+        try:
+            nml = os.path.join(work, "nml")
+            xm = 0.2 * n
+            with open(nml, "w") as f:
+                f.write("&harness\n nx=%d, ny=%d, nz=%d, xmax=%r, ymax=%r, zmax=%r, npd=13, icon=13,\n" % (
+                    n, n, n, xm, xm, xm))
+                f.write(" core4='OT', lsolid=.true., selec='Xe_He', r_cutoff=2.55d0, umax=2793.9365463d0,\n")
+                f.write(" mimpur=131.29d0, deltatps=5.d-4, niter=%d, pener=100000, outdir='%s', n4=1000,\n" % (
+                    steps, work))
+                f.write(" nthread=%d, ldump_init=.false., mode=-1, nrk=0\n/\n" % nthreads)
+            env = dict(os.environ, MKL_THREADING_LAYER="INTEL", OMP_NUM_THREADS=str(nthreads),
+                       MKL_NUM_THREADS=str(nthreads), KMP_DUPLICATE_LIB_OK="TRUE")
+            with open(nml) as fin:
+                subprocess.run([harness], stdin=fin, cwd=work, env=env, check=True, timeout=600,
+                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            with open(os.path.join(work, "timing.txt")) as f:
+                sec, nst = f.read().split()
+            sec, nst = float(sec), int(nst)
+            return {"value": n ** 3 * nst / sec / 1e6, "unit": "Mpoint-steps/s", "cores": nthreads,
+                    "kind": "reference",
+                    "sample": "%d predictor-corrector steps of He-1000+Xe %d^3 FP64, reference Fortran "
+                              "routines (flang -O2 -fopenmp, MKL FFTW3), %.1f s" % (nst, n, sec)}
+        except Exception as exc:  # fall through to the oracle
+            print("reference harness failed: %r" % (exc,), file=sys.stderr)
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    from hetd import capi
+    from hetd.initstate import woods_saxon_state
+    m = 128
+    p = capi.default_params(nx=m, ny=m, nz=m, xmax=0.2 * m, ymax=0.2 * m, zmax=0.2 * m, npd=13, icon=13)
+    o = oracle.OracleCtx(p, selec="Xe_He", r_cutoff=2.55, umax=2793.9365463)
+    o.set_state(woods_saxon_state(m, m, m, p.xmax, p.ymax, p.zmax, 1000, 3), (0, 0, 0), (0, 0, 2 * KTOPS))
+    o.init()
+    t0 = time.perf_counter()
+    o.steppc(5e-4 / KTOPS)
+    o.potenimp(o.imp()[0])
+    o.poten()
+    o.set_aimp(o.forceimp(o.imp()[0]) / (p.mimpur * 0.02061484))
+    sec = time.perf_counter() - t0
+    o.close()
+    del np
+    return {"value": m ** 3 / sec / 1e6, "unit": "Mpoint-steps/s", "cores": 1, "kind": "port",
+            "sample": "1 predictor-corrector step of He-1000+Xe %d^3 FP64, C oracle, %.1f s" % (m, sec)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--kernel", default="update", choices=["update", "xpass", "ypass", "zpass", "force"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the ensemble split across "
+              "GPUs is described in DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    warmup = max(args.warmup, 3)
+
+    import numpy as np
+    from hetd import Context, capi
+    from hetd.initstate import woods_saxon_state
+
+    n = args.n
+    xm = 0.2 * n
+    p = capi.default_params(nx=n, ny=n, nz=n, xmax=xm, ymax=xm, zmax=xm, npd=13, icon=13)
+    t0 = time.perf_counter()
+    psi = woods_saxon_state(n, n, n, xm, xm, xm, 1000, 3)
+    ctx = Context(p, selec="Xe_He", r_cutoff=2.55, umax=2793.9365463)
+    ctx.set_state(psi, (0.0, 0.0, 0.0), (0.0, 0.0, 2.0 * KTOPS))
+    del psi
+    ctx.init()
+    dt = 5e-4 / KTOPS
+    ctx.evolve(dt, warmup, traj=False)
+    e0 = ctx.energy()
+    setup_s = time.perf_counter() - t0
+    print("setup %.1f s, E/N %.6f K" % (setup_s, e0["etot4"] / 1000.0), file=sys.stderr, flush=True)
+
+    kclass = {"update": capi.K_UPDATE, "xpass": capi.K_XPASS, "ypass": capi.K_YPASS,
+              "zpass": capi.K_ZPASS, "force": capi.K_FORCE}[args.kernel]
+    ev = HipEvents()
+    stream = ctx.stream()
+    a, b = ev.event(), ev.event()
+    ctx.timing(kclass)
+    ev.record(a, stream)
+    w0 = time.perf_counter()
+    ctx.evolve(dt, args.steps, traj=False)
+    ev.record(b, stream)
+    ms = ev.elapsed_ms(a, b)
+    wall = time.perf_counter() - w0
+    kms, klaunch, kbytes = ctx.timing_read()
+    ctx.timing(capi.K_OFF)
+
+    # per-class breakdown (short extra runs, one class at a time)
+    classes = {}
+    for kc, name in capi.K_NAMES.items():
+        ctx.timing(kc)
+        ctx.evolve(dt, 2, traj=False)
+        t, nl, by = ctx.timing_read()
+        classes[name] = {"ms_per_step": round(t / 2, 4), "launches_per_step": nl // 2,
+                         "GBps": round(by / (t * 1e-3) / 1e9, 1) if t > 0 else None}
+    ctx.timing(capi.K_OFF)
+    e1 = ctx.energy()
+    ctx.close()
+
+    pts = float(n) ** 3
+    ms_step = ms / args.steps
+    value = pts * args.steps / (ms * 1e-3) / 1e6
+    avg_launch_s = kms * 1e-3 / max(klaunch, 1)
+    achieved = (kbytes / max(klaunch, 1)) / avg_launch_s / 1e9
+    step_bytes = model_bytes_per_point_step(n) * pts
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s_fp64_%d.json" % (args.kernel, n))
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    cpu = None
+    if not args.no_cpu:
+        nth = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(n, nth)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpoint-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Woods-Saxon He-1000 droplet, seeded phase, Xe at the centre moving 200 m/s)",
+        "config": {"workload": "He-1000+Xe %d^3 0.4A OT+solid npd13 icon13 dt0.5fs FP64 predictor-corrector "
+                               "steps" % n, "grid": [n, n, n], "precision": "fp64",
+                   "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
+                   "step_hbm_roof_frac": round(step_bytes / (ms_step * 1e-3) / HBM_PEAK, 4),
+                   "model_bytes_per_point_step": round(model_bytes_per_point_step(n), 2),
+                   "dE_over_E_during_timed": abs(e1["etot"] - e0["etot"]) / abs(e0["etot"]),
+                   "kernel_classes": classes},
+        "roofline": {"bound": "hbm", "kernel": args.kernel, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4),
+                     "traffic": traffic, "launches": klaunch,
+                     "bytes_per_launch": kbytes / max(klaunch, 1),
+                     "avg_launch_us": round(avg_launch_s * 1e6, 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

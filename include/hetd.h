@@ -167,6 +167,22 @@ int hetd_deriv(hetd_ctx *ctx, int32_t id, int32_t iv, int32_t is_complex,
  * order their own work against the library's. */
 void *hetd_stream(hetd_ctx *ctx);
 
+/* Kernel-class timing with HIP events on the context stream.  No counterpart
+ * in the reference (which only times its whole loop, DFT4He3d.f90:829-831,
+ * 1006-1015); it serves benchmarks and roofline accounting.  While kclass is
+ * nonzero every launch of that class is bracketed by a pair of events and
+ * its algorithmic HBM bytes (one read of each input, one write of each
+ * output) are accumulated; hetd_timing_read returns the summed event time,
+ * the launch count and the bytes since the last hetd_timing call. */
+#define HETD_K_OFF 0
+#define HETD_K_UPDATE 1 /* fused stencil + right-hand side + integrator update */
+#define HETD_K_XPASS 2  /* x-axis FFT pass with its fused pointwise work */
+#define HETD_K_YPASS 3  /* y-axis FFT pass */
+#define HETD_K_ZPASS 4  /* z-axis FFT pass with the fused spectral multiply */
+#define HETD_K_FORCE 5  /* density gradient + impurity-force reduction */
+int hetd_timing(hetd_ctx *ctx, int32_t kclass);
+int hetd_timing_read(hetd_ctx *ctx, double *total_ms, int64_t *launches, double *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -776,6 +776,11 @@ int orc_steppc(orc_ctx *c, double dt, double *errHe, double *errimp) {
     return 0;
 }
 
+int orc_set_aimp(orc_ctx *c, const double a[3]) {
+    for (int d = 0; d < 3; ++d) c->aimp[d] = a[d];
+    return 0;
+}
+
 int orc_init(orc_ctx *c) {
     potential_update(c, c->rimp);
     return 0;

@@ -32,6 +32,7 @@ int orc_init(orc_ctx *c); /* first potenimp/poten/forceimp/energy (DFT4He3d.f90:
 int orc_potenimp(orc_ctx *c, const double r[3]);
 int orc_poten(orc_ctx *c);
 int orc_forceimp(orc_ctx *c, const double r[3], double F[3]);
+int orc_set_aimp(orc_ctx *c, const double a[3]); /* the driver's aimp = F/mAg_u */
 int orc_steprk(orc_ctx *c, double dt);
 int orc_steppc(orc_ctx *c, double dt, double *errHe, double *errimp);
 int orc_energy(orc_ctx *c, hetd_energies *e);

@@ -46,6 +46,7 @@ def lib():
         L.orc_potenimp.argtypes = [_vp, _dp]
         L.orc_poten.argtypes = [_vp]
         L.orc_forceimp.argtypes = [_vp, _dp, _dp]
+        L.orc_set_aimp.argtypes = [_vp, _dp]
         L.orc_steprk.argtypes = [_vp, C.c_double]
         L.orc_steppc.argtypes = [_vp, C.c_double, _dp, _dp]
         L.orc_energy.argtypes = [_vp, C.POINTER(capi.Energies)]
@@ -135,6 +136,9 @@ class OracleCtx:
         F = np.zeros(3)
         self._chk(self.L.orc_forceimp(self.h, _p(_v3(r)), _p(F)))
         return F
+
+    def set_aimp(self, a):
+        self._chk(self.L.orc_set_aimp(self.h, _p(_v3(a))))
 
     def steprk(self, dt):
         self._chk(self.L.orc_steprk(self.h, dt))

@@ -1,0 +1,56 @@
+"""The C ABI library loads and exports every entry point of include/hetd.h
+(no compute calls: runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from hetd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    with open(os.path.join(ROOT, "include", "hetd.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hetd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert set(declared()) == set(capi.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.fail("lib/libhetd.so is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    for name in declared():
+        assert hasattr(lib, name), name
+
+
+def test_host_side_setup_entry_points():
+    # pure host functions of the ABI: impurity potential and its table
+    L = capi.load()
+    v = ctypes.c_double()
+    assert L.hetd_vimp(b"Xe_He", 2.0, 1.0, 0.0, ctypes.byref(v)) == 0
+    assert abs(v.value - 21450.4867067095) < 1e-9
+    assert L.hetd_vimp(b"Xe_He", 2.0, 2.55, 2793.9365463, ctypes.byref(v)) == 0
+    assert v.value == 2793.9365463
+    assert L.hetd_vimp(b"no_such_potential", 2.0, 2.0, 0.0, ctypes.byref(v)) == capi.HETD_ERR_ARG
+    from hetd import build_potion
+    tab, d = build_potion("Xe_He", 2.55, 2793.9365463, 32, 16.0)
+    assert len(tab) == 3201 and abs(d - 64.0 / 3200) < 1e-15
+    import oracle
+    otab, od = oracle.build_potion("Xe_He", 2.55, 2793.9365463, 32, 16.0)
+    assert od == d and (abs(otab - tab) <= 1e-13 * abs(otab).max()).all()
+
+
+def test_default_params_match_python_mirror():
+    L = capi.load()
+    p = capi.Params()
+    L.hetd_default_params(ctypes.byref(p))
+    q = capi.default_params()
+    for name, _ in capi.Params._fields_:
+        assert getattr(p, name) == getattr(q, name), name

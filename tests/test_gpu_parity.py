@@ -1,0 +1,193 @@
+"""Parity of the HIP path (lib/libhetd.so on gfx950) with the oracle and with
+the reference's own golden vectors.  FP64 bar: 1e-10 relative."""
+import sys
+
+import numpy as np
+import pytest
+
+import golden_util as G
+from hetd import Context, capi
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_pair(oracle_mod, p, selec="null", r_cutoff=2.0, umax=0.0):
+    ctx = Context(p, selec=selec, r_cutoff=r_cutoff, umax=umax)
+    orc = oracle_mod.OracleCtx(p, selec=selec, r_cutoff=r_cutoff, umax=umax)
+    return ctx, orc
+
+
+# --------------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize("shape", [(8, 6, 10), (27, 10, 12), (16, 16, 16), (5, 9, 45), (20, 12, 36),
+                                   (32, 24, 64), (40, 30, 50)])
+def test_fft_matches_numpy_and_oracle(oracle_mod, shape):
+    say("fft", shape)
+    nz, ny, nx = shape
+    p = capi.default_params(nx=nx, ny=ny, nz=nz, xmax=nx * 0.2, ymax=ny * 0.2, zmax=nz * 0.2, npd=3)
+    with Context(p) as ctx:
+        rng = np.random.default_rng(7)
+        f = rng.standard_normal(shape)
+        F = ctx.fft_r2c(f)
+        ref = np.fft.rfftn(f)
+        assert np.abs(F - ref).max() < 1e-12 * np.abs(ref).max()
+        assert np.abs(F - oracle_mod.fft_r2c(f)).max() < 1e-12 * np.abs(ref).max()
+        g = ctx.fft_c2r(F)  # includes *renor (fft.f90:108)
+        assert np.abs(g - f).max() < 1e-13 * np.abs(f).max()
+        # non-Hermitian input: one-dimensional c2r semantics along x
+        H = F + 0.3 * (rng.standard_normal(F.shape) + 1j * rng.standard_normal(F.shape))
+        want = oracle_mod.fft_c2r(H, nx) / f.size
+        assert np.abs(ctx.fft_c2r(H) - want).max() < 1e-12 * np.abs(want).max()
+
+
+# ------------------------------------------------------------- stencils
+
+@pytest.mark.parametrize("npd,icon", [(13, 13), (9, 8), (13, 18), (3, 13), (25, 8)])
+def test_derivatives_match_oracle(oracle_mod, npd, icon):
+    say("deriv", npd, icon)
+    shape = (28, 27, 30)
+    p = capi.default_params(nx=30, ny=27, nz=28, xmax=6.0, ymax=5.4, zmax=5.6, npd=npd, icon=icon)
+    h = (0.4, 0.4, 0.4)
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal(shape)
+    fc = f + 1j * rng.standard_normal(shape)
+    with Context(p) as ctx:
+        for iv in (1, 2, 3):
+            for id_ in (1, 2):
+                for arr in (f, fc):
+                    got = ctx.deriv(arr, id_, iv)
+                    want = oracle_mod.deriv(arr, npd, icon, id_, iv, h[iv - 1])
+                    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max(), (iv, id_)
+
+
+# ------------------------------------------------- golden configurations
+
+@pytest.mark.parametrize("name", G.NAMES)
+def test_poten_energy_force_match_golden(oracle_mod, name):
+    say("golden init", name)
+    cfg, d = G.load(name)
+    p = G.params_for(cfg, capi)
+    ctx, orc = make_pair(oracle_mod, p, cfg["selec"], cfg["r_cutoff"], cfg["umax"])
+    ctx.set_state(d["psi0"], d["rimp0"], d["vimp0"])
+    ctx.init()
+    orc.set_state(d["psi0"], d["rimp0"], d["vimp0"])
+    orc.init()
+    for key, fid in (("pot4", capi.F_POT4), ("dencg", capi.F_DENCG), ("delj4", capi.F_DELJ4),
+                     ("ccor", capi.F_CCOR), ("dxden", capi.F_DXDEN), ("dyden", capi.F_DYDEN),
+                     ("dzden", capi.F_DZDEN), ("uimp", capi.F_UIMP), ("penalty", capi.F_PENALTY),
+                     ("ualphas", capi.F_UALPHAS), ("denalf", capi.F_DENALF)):
+        if "init_%s_planes" % key not in d:
+            continue
+        if key == "ccor" and cfg["core"] == "OTC":
+            continue  # term_alfa reuses sto1 (term_alfa.f90:83-86)
+        got = ctx.field(fid)
+        assert G.rel(G.planes(got), d["init_%s_planes" % key]) < TOL, key
+        assert G.rel(got, orc.field(fid)) < TOL, key
+    if cfg["core"] != "OTC":
+        assert G.rel(G.planes(ctx.field(capi.F_FDEN)), d["init_fden_planes"]) < 1e-12
+    assert G.rel(ctx.field(capi.F_FVLJ), orc.field(capi.F_FVLJ)) < 1e-15
+    assert G.rel(ctx.field(capi.F_WCGK), orc.field(capi.F_WCGK)) < 1e-15
+    e = ctx.energy()
+    ref = d["init_energy"]
+    for i, k in enumerate(("etot4", "ekin4", "elj4", "ecor4", "esolid", "ealphas", "ekinx", "eHeX", "etot")):
+        assert abs(e[k] - ref[i]) <= TOL * max(abs(ref[i]), 1.0), k
+    r, v, a = ctx.imp()
+    assert np.abs(a - d["init_aimp"]).max() <= TOL * max(np.abs(d["init_aimp"]).max(), 1e-300)
+    assert abs(ctx.norm() - orc.norm()) < 1e-13 * orc.norm()
+
+
+@pytest.mark.parametrize("name", G.NAMES)
+def test_evolve_matches_golden_trajectory(oracle_mod, name):
+    say("golden evolve", name)
+    cfg, d = G.load(name)
+    p = G.params_for(cfg, capi)
+    with Context(p, selec=cfg["selec"], r_cutoff=cfg["r_cutoff"], umax=cfg["umax"]) as ctx:
+        ctx.set_state(d["psi0"], d["rimp0"], d["vimp0"])
+        ctx.init()
+        traj = ctx.evolve(G.DT, cfg["niter"])
+        st = d["steps"]
+        psimax = np.abs(d["psi0"]).max()
+        assert np.all(np.abs(traj[:, 0] - st[:, 1]) <= 1e-8 * np.abs(st[:, 1]) + 1e-15 * psimax)
+        assert np.all(np.abs(traj[:, 1] - st[:, 2]) <= 1e-6 * np.abs(st[:, 2]) + 1e-15)
+        scale = np.maximum(np.abs(st[:, 3:12]).max(axis=0), 1e-30)
+        assert (np.abs(traj[:, 2:] - st[:, 3:12]).max(axis=0) / scale).max() < TOL
+        assert G.rel(G.planes(ctx.field(capi.F_PSI)), d["final_planes"]) < TOL
+        en = d["energy"][-1]
+        e = ctx.energy()
+        assert abs(ctx.norm() - en[1]) < TOL * en[1]
+        assert abs(e["etot4"] - en[2]) < TOL * abs(en[2])
+        assert abs(e["etot"] - en[10]) < TOL * abs(en[10])
+
+
+def test_fine_grained_calls_match_oracle(oracle_mod):
+    """steprk x3 / steppc through the per-routine entry points, the driver's
+    potenimp/poten/forceimp/aimp sequence done by the caller."""
+    say("fine-grained")
+    cfg, d = G.load("ot_xe_32")
+    p = G.params_for(cfg, capi)
+    ctx, orc = make_pair(oracle_mod, p, cfg["selec"], cfg["r_cutoff"], cfg["umax"])
+    for c in (ctx, orc):
+        c.set_state(d["psi0"], d["rimp0"], d["vimp0"])
+        c.init()
+    m = p.mimpur * 0.02061484
+    for it in range(6):
+        if it < 3:
+            ctx.steprk(G.DT)
+            orc.steprk(G.DT)
+            e1 = e2 = (0.0, 0.0)
+        else:
+            e1 = ctx.steppc(G.DT)
+            e2 = orc.steppc(G.DT)
+        for c in (ctx, orc):
+            r, v, a = c.imp()
+            c.potenimp(r)
+            c.poten()
+            F = c.forceimp(r)
+            c.set_aimp(F / m)
+        assert abs(e1[0] - e2[0]) <= 1e-8 * abs(e2[0]) + 1e-16
+        assert G.rel(ctx.field(capi.F_PSI), orc.field(capi.F_PSI)) < TOL
+        assert G.rel(ctx.field(capi.F_POT4), orc.field(capi.F_POT4)) < TOL
+        assert G.rel(ctx.field(capi.F_PC), orc.field(capi.F_PC)) < 1e-8
+        for x, y in zip(ctx.imp(), orc.imp()):
+            assert np.abs(x - y).max() <= TOL * max(np.abs(y).max(), 1e-300)
+
+
+@pytest.mark.parametrize("shape,npd,icon,core", [((48, 40, 36), 13, 13, "OT"), ((30, 32, 45), 9, 8, "OT"),
+                                                 ((36, 36, 36), 13, 18, "OTC")])
+def test_evolve_matches_oracle_other_grids(oracle_mod, shape, npd, icon, core):
+    say("oracle evolve", shape, npd, icon, core)
+    from hetd.initstate import woods_saxon_state
+    nz, ny, nx = shape
+    h = 0.6
+    p = capi.default_params(nx=nx, ny=ny, nz=nz, xmax=nx * h / 2, ymax=ny * h / 2, zmax=nz * h / 2, npd=npd,
+                            icon=icon, core=core, lsolid=0 if core == "OTC" else 1, tzmean=nz * h / 2 - 2.0)
+    psi = woods_saxon_state(nx, ny, nz, p.xmax, p.ymax, p.zmax, 150, 3)
+    ctx, orc = make_pair(oracle_mod, p, "Xe_He", 2.55, 2793.9365463)
+    for c in (ctx, orc):
+        c.set_state(psi, (0.3, -0.2, 0.1), (1.0, 2.0, -3.0))
+        c.init()
+    t1 = ctx.evolve(G.DT, 7)
+    t2 = orc.evolve(G.DT, 7)
+    assert np.abs(t1[:, 2:] - t2[:, 2:]).max() <= TOL * np.abs(t2[:, 2:]).max()
+    assert G.rel(ctx.field(capi.F_PSI), orc.field(capi.F_PSI)) < TOL
+    e1, e2 = ctx.energy(), orc.energy()
+    for k in e2:
+        assert abs(e1[k] - e2[k]) <= TOL * max(abs(e2[k]), 1.0), k
+
+
+def test_guard_reports_nonpositive_density():
+    say("guard")
+    p = capi.default_params(nx=16, ny=16, nz=16, xmax=4.0, ymax=4.0, zmax=4.0, npd=9)
+    psi = np.full((16, 16, 16), 0.1 + 0j)
+    psi[5, 5, 5] = np.nan
+    with Context(p) as ctx:
+        ctx.set_state(psi, (0, 0, 0), (0, 0, 0))
+        ctx.init()
+        with pytest.raises(RuntimeError, match="den not > 0"):
+            ctx.steprk(G.DT)
