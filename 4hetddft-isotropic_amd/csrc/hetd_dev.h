@@ -226,14 +226,20 @@ struct ImpState {
     int pad;
 };
 
-// u_imp(x) = P[i](1-w) + P[i+1] w, r = |x - R|, i = int(r/D), w = mod(r,D)/D
-// (potenimp.f90:82-93); the index is clamped to the table.
+// u_imp(x) = P[i](1-w) + P[i+1] w, r = |x - R|, i = int(r/D) from the
+// rounded quotient, w = mod(r,D)/D from the exact remainder (potenimp.f90:
+// 82-93).  Where r/D rounds up to an integer the two disagree and the
+// reference takes w ~ 1 at the upper node; the same is done here: the exact
+// r - i*D is negative exactly then, and mod adds one D back.  The index is
+// clamped to the table.
 __device__ __forceinline__ double uimp_at(const double* __restrict__ potion, int npot, double D,
                                           double xx, double yt) {
     const double r = sqrt(xx * xx + yt);
     int ir = (int)(r / D);
     ir = ir < 0 ? 0 : (ir > npot - 2 ? npot - 2 : ir);  // also catches a non-finite r
-    const double w = fma(-(double)ir, D, r) / D;  // mod(r, D)/D
+    double rem = fma(-(double)ir, D, r);
+    if (rem < 0.0) rem += D;
+    const double w = rem / D;
     return potion[ir] * (1.0 - w) + potion[ir + 1] * w;
 }
 
