@@ -39,6 +39,11 @@ struct hetd_ctx {
     double *dxden = nullptr, *dyden = nullptr, *dzden = nullptr, *denalf = nullptr, *falfs = nullptr;
     double *intx = nullptr, *inty = nullptr, *intz = nullptr, *ualphas = nullptr;
     double *s1 = nullptr, *s2 = nullptr, *s3 = nullptr, *s4 = nullptr, *s5 = nullptr, *s6 = nullptr;
+    // mixed precision (HETD_MIXED): FP32 density, mean field and FFT work
+    bool mixed = false;
+    float *den32 = nullptr, *pot4_32 = nullptr, *dencg32 = nullptr, *delj4_32 = nullptr, *tmp32 = nullptr;
+    float2 *A32 = nullptr, *B32 = nullptr, *C32 = nullptr;
+    float *fvlj32 = nullptr, *wcgk32 = nullptr, *kalfs32 = nullptr;
     double* potion = nullptr;
     int npot = 0;
     double delinter = 0;
@@ -118,52 +123,54 @@ static void tend(hetd_ctx* c, int k, double bytes) {
     c->tbytes += bytes;
 }
 
-// launches with their algorithmic HBM bytes (FP64: 8 B real, 16 B complex
-// per point; half spectra have nxh*ny*nz points)
-static hipError_t run_x(hetd_ctx* c, XPassArgs<double> x) {
-    const double n = (double)c->n, nh = (double)c->nh;
+// launches with their algorithmic HBM bytes (one read of each input, one
+// write of each output; half spectra have nxh*ny*nz points)
+template <typename T> static hipError_t run_x(hetd_ctx* c, XPassArgs<T> x) {
+    const double n = (double)c->n, nh = (double)c->nh, s = (double)sizeof(T);
     double b = 0;
     switch (x.mode) {
-    case XMODE_R2C: b = 8 * n + 16 * nh; break;
-    case XMODE_C2R: b = 16 * nh + 8 * n; break;
-    case XMODE_CG_S: b = 16 * nh + 8 * n + 8 * n + 16 * nh; break;
-    default: b = 16 * nh + 24 * n + 8 * n + (x.rout2 ? 8 * n : 0); break;
+    case XMODE_R2C: b = s * n + 2 * s * nh; break;
+    case XMODE_C2R: b = 2 * s * nh + s * n; break;
+    case XMODE_CG_S: b = 4 * s * nh + 2 * s * n; break;
+    default: b = 2 * s * nh + 4 * s * n + (x.rout2 ? s * n : 0); break;
     }
     tbeg(c, HETD_K_XPASS);
-    hipError_t e = launch_xpass(x, c->st);
+    hipError_t e = reg_fft_supported(x.ax.n) ? launch_xreg(x, c->st) : launch_xpass(x, c->st);
     tend(c, HETD_K_XPASS, b);
     return e;
 }
 
-static hipError_t run_y(hetd_ctx* c, YPassArgs<double> y, int nbuf) {
+template <typename T> static hipError_t run_y(hetd_ctx* c, YPassArgs<T> y, int nbuf) {
     tbeg(c, HETD_K_YPASS);
-    hipError_t e = launch_ypass(y, c->nz, nbuf, c->st);
-    tend(c, HETD_K_YPASS, 32.0 * c->nh * nbuf);
+    hipError_t e = reg_fft_supported(y.ax.n) ? launch_yreg(y, c->nz, nbuf, c->st) : launch_ypass(y, c->nz, nbuf, c->st);
+    tend(c, HETD_K_YPASS, 4.0 * sizeof(T) * c->nh * nbuf);
     return e;
 }
 
-static hipError_t run_z(hetd_ctx* c, ZPassArgs<double> z, int nbuf) {
-    double per = 32.0;
-    if (z.mode == ZMODE_CONV) per = 16.0 + 24.0 * z.nk + (z.fwd_out ? 16.0 : 0.0);
+template <typename T> static hipError_t run_z(hetd_ctx* c, ZPassArgs<T> z, int nbuf) {
+    const double s = (double)sizeof(T);
+    double per = 4 * s;
+    if (z.mode == ZMODE_CONV) per = 2 * s + 3 * s * z.nk + (z.fwd_out ? 2 * s : 0.0);
     tbeg(c, HETD_K_ZPASS);
-    hipError_t e = launch_zpass(z, nbuf, c->st);
+    hipError_t e = reg_fft_supported(z.ax.n) ? launch_zreg(z, nbuf, c->st) : launch_zpass(z, nbuf, c->st);
     tend(c, HETD_K_ZPASS, per * c->nh * nbuf);
     return e;
 }
 
 static hipError_t run_update(hetd_ctx* c, const UpdateArgs& u, int* nb) {
-    double per = 144.0;
+    double per = 144.0;  // PRED / CORR (FP64 den and pot4)
     if (u.kind == STAGE_RK) per = u.rk_j == 0 ? 96.0 : 80.0;
+    if (u.denf) per -= 8.0;  // FP32 den write and pot4 read
     tbeg(c, HETD_K_UPDATE);
-    hipError_t e = launch_update(u, c->st, nb);
+    hipError_t e = tiled_supported(c->sc) ? launch_update_tiled(u, c->st, nb) : launch_update(u, c->st, nb);
     tend(c, HETD_K_UPDATE, per * c->n);
     return e;
 }
 
 static hipError_t run_force(hetd_ctx* c, const ForceArgs& f, int* nb) {
     tbeg(c, HETD_K_FORCE);
-    hipError_t e = launch_force_partial(f, c->st, nb);
-    tend(c, HETD_K_FORCE, 8.0 * c->n);
+    hipError_t e = tiled_supported(c->sc) ? launch_force_tiled(f, c->st, nb) : launch_force_partial(f, c->st, nb);
+    tend(c, HETD_K_FORCE, (f.denf ? 4.0 : 8.0) * c->n);
     return e;
 }
 
@@ -267,59 +274,82 @@ static int term_alfa_dev(hetd_ctx* c) {
     return 0;
 }
 
-// poten() (poten.f90:5-118): den -> pot4 (and dencg, delj4; OTC: denalf,
-// ualphas).  want_ccor / want_fden also store the correlation term in tmp
-// and the forward spectrum of den in Cs.
-static int poten_dev(hetd_ctx* c, bool want_ccor, bool want_fden) {
+// the density / mean-field / FFT buffers of one precision
+template <typename T> struct PF {
+    T *den, *pot4, *dencg, *delj4, *tmp;
+    cplx<T> *A, *B, *Cs;
+    const T *fvlj, *wcgk, *kalfs;
+};
+static PF<double> pf64(hetd_ctx* c) {
+    return PF<double>{c->den, c->pot4, c->dencg, c->delj4, c->tmp, c->A, c->B, c->Cs, c->fvlj, c->wcgk, c->kalfs};
+}
+static PF<float> pf32(hetd_ctx* c) {
+    return PF<float>{c->den32, c->pot4_32, c->dencg32, c->delj4_32, c->tmp32, c->A32, c->B32, c->C32,
+                     c->fvlj32, c->wcgk32, c->kalfs32};
+}
+
+// poten() (poten.f90:5-118) up to pot4 without the alpha_s term: den ->
+// pot4, dencg, delj4 (and, for OTC, the Gaussian spectrum in Cs).
+// want_ccor / want_fden also store the correlation term in tmp and the
+// forward spectrum of den in Cs.
+template <typename T>
+static int poten_t(hetd_ctx* c, const PF<T>& f, bool want_ccor, bool want_fden) {
     const long nl = (long)c->ny * c->nz;
-    XPassArgs<double> x{};
-    x.ax = c->ax[0]; x.nlines = nl; x.mode = XMODE_R2C; x.rin = c->den; x.cout = c->A;
+    XPassArgs<T> x{};
+    x.ax = c->ax[0]; x.nlines = nl; x.mode = XMODE_R2C; x.rin = f.den; x.cout = f.A;
     HCHK(c, run_x(c, x));
-    YPassArgs<double> y{};
-    y.ax = c->ax[1]; y.nxh = c->nxh; y.inv = 0; y.buf[0] = c->A;
+    YPassArgs<T> y{};
+    y.ax = c->ax[1]; y.nxh = c->nxh; y.inv = 0; y.buf[0] = f.A;
     HCHK(c, run_y(c, y, 1));
     // fden*W -> A, fden*V_LJ -> B [, fden*K_alpha -> Cs] (poten.f90:34-50, 96-101)
-    ZPassArgs<double> z{};
-    z.ax = c->ax[2]; z.nxh = c->nxh; z.ny = c->ny; z.mode = ZMODE_CONV; z.in[0] = c->A;
+    ZPassArgs<T> z{};
+    z.ax = c->ax[2]; z.nxh = c->nxh; z.ny = c->ny; z.mode = ZMODE_CONV; z.in[0] = f.A;
     z.nk = c->otc ? 3 : 2;
-    z.out[0][0] = c->A; z.out[0][1] = c->B; z.out[0][2] = c->Cs;
-    z.kern[0] = c->wcgk; z.kern[1] = c->fvlj; z.kern[2] = c->kalfs;
-    z.fwd_out = want_fden ? c->Cs : nullptr;
+    z.out[0][0] = f.A; z.out[0][1] = f.B; z.out[0][2] = f.Cs;
+    z.kern[0] = f.wcgk; z.kern[1] = f.fvlj; z.kern[2] = f.kalfs;
     if (want_fden && c->otc) return fail(c, HETD_ERR_STATE, "fden copy not available with OTC");
+    z.fwd_out = want_fden ? f.Cs : nullptr;
     HCHK(c, run_z(c, z, 1));
-    y.inv = 1; y.buf[0] = c->A; y.buf[1] = c->B; y.buf[2] = c->Cs;
+    y.inv = 1; y.buf[0] = f.A; y.buf[1] = f.B; y.buf[2] = f.Cs;
     HCHK(c, run_y(c, y, c->otc ? 3 : 2));
     // delj4 = c2r(fden*V)*renor
-    XPassArgs<double> xl{};
-    xl.ax = c->ax[0]; xl.nlines = nl; xl.mode = XMODE_C2R; xl.cin = c->B; xl.rout = c->delj4; xl.scale = c->renor;
+    XPassArgs<T> xl{};
+    xl.ax = c->ax[0]; xl.nlines = nl; xl.mode = XMODE_C2R; xl.cin = f.B; xl.rout = f.delj4; xl.scale = (T)c->renor;
     HCHK(c, run_x(c, xl));
     // dencg = c2r(fden*W)*renor, s = den*dencg*(cp4+cpp4*dencg), A = x-r2c(s)
-    XPassArgs<double> xc{};
-    xc.ax = c->ax[0]; xc.nlines = nl; xc.mode = XMODE_CG_S; xc.cin = c->A; xc.rin = c->den; xc.rout = c->dencg;
-    xc.cout = c->A; xc.scale = c->renor; xc.cp4 = c->p.cp4; xc.cpp4 = c->p.cpp4;
+    XPassArgs<T> xc{};
+    xc.ax = c->ax[0]; xc.nlines = nl; xc.mode = XMODE_CG_S; xc.cin = f.A; xc.rin = f.den; xc.rout = f.dencg;
+    xc.cout = f.A; xc.scale = (T)c->renor; xc.cp4 = (T)c->p.cp4; xc.cpp4 = (T)c->p.cpp4;
     HCHK(c, run_x(c, xc));
-    y.inv = 0; y.buf[0] = c->A;
+    y.inv = 0; y.buf[0] = f.A;
     HCHK(c, run_y(c, y, 1));
-    ZPassArgs<double> z2{};
-    z2.ax = c->ax[2]; z2.nxh = c->nxh; z2.ny = c->ny; z2.mode = ZMODE_CONV; z2.nk = 1; z2.in[0] = c->A;
-    z2.out[0][0] = c->A; z2.kern[0] = c->wcgk;
+    ZPassArgs<T> z2{};
+    z2.ax = c->ax[2]; z2.nxh = c->nxh; z2.ny = c->ny; z2.mode = ZMODE_CONV; z2.nk = 1; z2.in[0] = f.A;
+    z2.out[0][0] = f.A; z2.kern[0] = f.wcgk;
     HCHK(c, run_z(c, z2, 1));
     y.inv = 1;
     HCHK(c, run_y(c, y, 1));
     // pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + sto1 [+ penalty] (poten.f90:80-94, 110-114)
-    XPassArgs<double> xp{};
-    xp.ax = c->ax[0]; xp.nlines = nl; xp.mode = XMODE_POT; xp.cin = c->A; xp.rin = c->den; xp.rin2 = c->dencg;
-    xp.rin3 = c->delj4; xp.rout = c->pot4; xp.rout2 = want_ccor ? c->tmp : nullptr; xp.scale = c->renor;
-    xp.a0 = c->p.cp4 / 2.0; xp.a1 = c->p.cpp4 / 3.0;
+    XPassArgs<T> xp{};
+    xp.ax = c->ax[0]; xp.nlines = nl; xp.mode = XMODE_POT; xp.cin = f.A; xp.rin = f.den; xp.rin2 = f.dencg;
+    xp.rin3 = f.delj4; xp.rout = f.pot4; xp.rout2 = want_ccor ? f.tmp : nullptr; xp.scale = (T)c->renor;
+    xp.a0 = (T)(c->p.cp4 / 2.0); xp.a1 = (T)(c->p.cpp4 / 3.0);
     xp.lsolid = c->p.lsolid && !c->otc;
-    xp.csol = c->p.c_solid; xp.beta = c->p.beta_solid; xp.den_m = c->p.den_m;
+    xp.csol = (T)c->p.c_solid; xp.beta = (T)c->p.beta_solid; xp.den_m = (T)c->p.den_m;
     HCHK(c, run_x(c, xp));
+    return 0;
+}
+
+static int poten_dev(hetd_ctx* c, bool want_ccor, bool want_fden) {
+    if (c->mixed) return poten_t<float>(c, pf32(c), want_ccor, want_fden);
+    int rc = poten_t<double>(c, pf64(c), want_ccor, want_fden);
+    if (rc) return rc;
     if (c->otc) {
         XPassArgs<double> xa{};
-        xa.ax = c->ax[0]; xa.nlines = nl; xa.mode = XMODE_C2R; xa.cin = c->Cs; xa.rout = c->denalf;
+        xa.ax = c->ax[0]; xa.nlines = (long)c->ny * c->nz; xa.mode = XMODE_C2R; xa.cin = c->Cs; xa.rout = c->denalf;
         xa.scale = c->renor;
         HCHK(c, run_x(c, xa));
-        int rc = term_alfa_dev(c);
+        rc = term_alfa_dev(c);
         if (rc) return rc;
     }
     return 0;
@@ -329,6 +359,7 @@ static int poten_dev(hetd_ctx* c, bool want_ccor, bool want_fden) {
 static int force_dev(hetd_ctx* c) {
     ForceArgs f{};
     f.g = c->grid; f.sc = c->sc; f.den = c->den; f.imp = c->imp; f.partial = c->partial;
+    if (c->mixed) f.denf = c->den32;
     int nb = 0;
     HCHK(c, run_force(c, f, &nb));
     HCHK(c, launch_force_final(c->imp, c->partial, nb, c->dxyz, c->mAg, c->st));
@@ -345,6 +376,7 @@ static UpdateArgs update_base(hetd_ctx* c, double dt) {
     UpdateArgs u{};
     u.g = c->grid; u.sc = c->sc; u.h2o2m4 = c->p.h2o2m4; u.dt = dt; u.psi = c->psi; u.pot4 = c->pot4;
     u.imp = c->imp; u.den = c->den; u.guard = c->guard; u.partial = c->partial;
+    if (c->mixed) { u.pot4f = c->pot4_32; u.denf = c->den32; }
     return u;
 }
 
@@ -491,7 +523,10 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
     if (p->nx < 1 || p->ny < 1 || p->nz < 1) rc = bad("grid sizes must be positive");
     else if (p->npd < 3 || p->npd > 25 || p->npd % 2 == 0) rc = bad("npd must be odd, 3..25 (newder.f:75-76)");
     else if (p->icon != 13 && p->icon != 8 && p->icon != 18) rc = bad("icon must be 13, 8 or 18");
-    else if (p->precision != HETD_FP64) rc = bad("only HETD_FP64 is available in this build");
+    else if (p->precision != HETD_FP64 && p->precision != HETD_MIXED) rc = bad("precision must be HETD_FP64 or HETD_MIXED");
+    else if (p->precision == HETD_MIXED && !p->lsolid && p->core == HETD_CORE_OTC)
+        rc = bad("HETD_MIXED supports the OT core (the alpha_s term runs in FP64 only)");
+    else if (p->precision == HETD_MIXED && p->npd != 9 && p->npd != 13) rc = bad("HETD_MIXED needs npd 9 or 13");
     else if (p->nx > 1024 || p->ny > 1024 || p->nz > 1024) rc = bad("grid edge above 1024");
     if (rc) { *out = c; return rc; }
     c->nx = p->nx; c->ny = p->ny; c->nz = p->nz; c->nxh = p->nx / 2 + 1;
@@ -504,6 +539,7 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
     c->mAg = p->mimpur * 0.02061484;  // mp_u (modules.f90:315)
     c->renor = 1.0 / (double)(c->nx * c->ny * c->nz);  // fft.f90:36
     c->otc = (!p->lsolid && p->core == HETD_CORE_OTC);
+    c->mixed = p->precision == HETD_MIXED;
     *out = c;
     const int nn[3] = {c->nx, c->ny, c->nz};
     for (int a = 0; a < 3; ++a) {
@@ -573,6 +609,20 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
                           &c->ualphas, &c->s1, &c->s2, &c->s3, &c->s4, &c->s5, &c->s6};
         for (auto a : arr) HCHK(c, dalloc(c, a, nR));
     }
+    if (c->mixed) {
+        const size_t fR = sizeof(float) * c->n, fH = sizeof(float2) * c->nh, fK = sizeof(float) * c->nh;
+        HCHK(c, dalloc(c, &c->den32, fR));
+        HCHK(c, dalloc(c, &c->pot4_32, fR));
+        HCHK(c, dalloc(c, &c->dencg32, fR));
+        HCHK(c, dalloc(c, &c->delj4_32, fR));
+        HCHK(c, dalloc(c, &c->tmp32, fR));
+        HCHK(c, dalloc(c, &c->A32, fH));
+        HCHK(c, dalloc(c, &c->B32, fH));
+        HCHK(c, dalloc(c, &c->C32, fH));
+        HCHK(c, dalloc(c, &c->fvlj32, fK));
+        HCHK(c, dalloc(c, &c->wcgk32, fK));
+        HCHK(c, dalloc(c, &c->kalfs32, fK));
+    }
     HCHK(c, dalloc(c, &c->imp, sizeof(ImpState)));
     HCHK(c, dalloc(c, &c->guard, sizeof(int)));
     const long nblocks = (long)((c->nx + 63) / 64) * ((c->ny + 3) / 4) * c->nz;
@@ -604,6 +654,12 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
         HCHK(c, hipMemcpy(c->fvlj, fv.data(), sizeof(double) * c->nh, hipMemcpyHostToDevice));
         HCHK(c, hipMemcpy(c->wcgk, wc.data(), sizeof(double) * c->nh, hipMemcpyHostToDevice));
         HCHK(c, hipMemcpy(c->kalfs, ka.data(), sizeof(double) * c->nh, hipMemcpyHostToDevice));
+        if (c->mixed) {
+            HCHK(c, launch_to_float(c->fvlj, c->fvlj32, c->nh, c->st));
+            HCHK(c, launch_to_float(c->wcgk, c->wcgk32, c->nh, c->st));
+            HCHK(c, launch_to_float(c->kalfs, c->kalfs32, c->nh, c->st));
+            HCHK(c, hipStreamSynchronize(c->st));
+        }
     }
     fill_grid(c);
     return HETD_OK;
@@ -640,6 +696,7 @@ int hetd_set_state(hetd_ctx* c, const double* psi, const double r[3], const doub
     const size_t nC = sizeof(double2) * c->n;
     HCHK(c, hipMemcpyAsync(c->psi, psi, nC, hipMemcpyHostToDevice, c->st));
     HCHK(c, launch_den_from_psi(c->psi, c->den, c->n, c->st));
+    if (c->mixed) HCHK(c, launch_den32_from_psi(c->psi, c->den32, c->n, c->st));
     HCHK(c, hipMemsetAsync(c->q, 0, nC, c->st));
     HCHK(c, hipMemsetAsync(c->pc, 0, nC, c->st));
     for (int i = 0; i < 3; ++i) HCHK(c, hipMemsetAsync(c->psiold[i], 0, nC, c->st));
@@ -757,8 +814,9 @@ int hetd_energy(hetd_ctx* c, hetd_energies* e) {
     a.den_m = c->p.den_m; a.lsolid = c->p.lsolid; a.otc = c->otc ? 1 : 0;
     a.falfs = c->falfs; a.dxden = c->dxden; a.dyden = c->dyden; a.dzden = c->dzden;
     a.intx = c->intx; a.inty = c->inty; a.intz = c->intz; a.partial = c->partial;
+    if (c->mixed) { a.dencgf = c->dencg32; a.delj4f = c->delj4_32; }
     int nb = 0;
-    HCHK(c, launch_energy_partial(a, c->st, &nb));
+    HCHK(c, tiled_supported(c->sc) ? launch_energy_tiled(a, c->st, &nb) : launch_energy_partial(a, c->st, &nb));
     HCHK(c, launch_reduce_final(c->partial, nb, EN_NCOMP, c->red, c->st));
     double s[EN_NCOMP];
     ImpState is;
@@ -786,8 +844,9 @@ int hetd_norm(hetd_ctx* c, double* norm) {
     EnergyArgs a{};
     a.g = c->grid; a.sc = c->sc; a.psi = c->psi; a.den = c->den; a.dencg = c->dencg; a.delj4 = c->delj4;
     a.imp = c->imp; a.partial = c->partial;
+    if (c->mixed) { a.dencgf = c->dencg32; a.delj4f = c->delj4_32; }
     int nb = 0;
-    HCHK(c, launch_energy_partial(a, c->st, &nb));
+    HCHK(c, tiled_supported(c->sc) ? launch_energy_tiled(a, c->st, &nb) : launch_energy_partial(a, c->st, &nb));
     HCHK(c, launch_reduce_final(c->partial, nb, EN_NCOMP, c->red, c->st));
     double s[EN_NCOMP];
     HCHK(c, hipMemcpyAsync(s, c->red, sizeof s, hipMemcpyDeviceToHost, c->st));
@@ -805,18 +864,25 @@ int hetd_get_imp(hetd_ctx* c, double r[3], double v[3], double a[3]) {
     return HETD_OK;
 }
 
+// FP32 field of the mixed path widened into the FP64 scratch tmp
+static const double* widen(hetd_ctx* c, const float* f, long n, double* dst) {
+    if (launch_to_double(f, dst, n, c->st) != hipSuccess) return nullptr;
+    return dst;
+}
+
 int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
     if (!c || !host) return HETD_ERR_ARG;
     const size_t nR = sizeof(double) * c->n, nC = sizeof(double2) * c->n, nHr = sizeof(double) * c->nh;
     const void* src = nullptr;
     size_t bytes = nR;
     int rc = 0;
+    const bool mx = c->mixed;
     switch (which) {
     case HETD_F_PSI: src = c->psi; bytes = nC; break;
-    case HETD_F_DEN: src = c->den; break;
-    case HETD_F_POT4: src = c->pot4; break;
-    case HETD_F_DENCG: src = c->dencg; break;
-    case HETD_F_DELJ4: src = c->delj4; break;
+    case HETD_F_DEN: src = mx ? widen(c, c->den32, c->n, c->tmp) : c->den; break;
+    case HETD_F_POT4: src = mx ? widen(c, c->pot4_32, c->n, c->tmp) : c->pot4; break;
+    case HETD_F_DENCG: src = mx ? widen(c, c->dencg32, c->n, c->tmp) : c->dencg; break;
+    case HETD_F_DELJ4: src = mx ? widen(c, c->delj4_32, c->n, c->tmp) : c->delj4; break;
     case HETD_F_PC: src = c->pc; bytes = nC; break;
     case HETD_F_Q: src = c->q; bytes = nC; break;
     case HETD_F_FVLJ: src = c->fvlj; bytes = nHr; break;
@@ -828,13 +894,19 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
         if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
         if (which == HETD_F_FDEN && c->otc) return fail(c, HETD_ERR_ARG, "FDEN not kept with OTC");
         if ((rc = poten_dev(c, which == HETD_F_CCOR, which == HETD_F_FDEN))) return rc;
-        src = which == HETD_F_CCOR ? (const void*)c->tmp : (const void*)c->Cs;
-        bytes = which == HETD_F_CCOR ? nR : sizeof(double2) * c->nh;
+        if (which == HETD_F_CCOR) {
+            src = mx ? widen(c, c->tmp32, c->n, c->tmp) : (const void*)c->tmp;
+        } else {
+            src = mx ? widen(c, reinterpret_cast<const float*>(c->C32), 2 * c->nh, reinterpret_cast<double*>(c->A))
+                     : (const void*)c->Cs;
+            bytes = sizeof(double2) * c->nh;
+        }
         break;
     }
     case HETD_F_DXDEN:
     case HETD_F_DYDEN:
     case HETD_F_DZDEN:
+        if (mx) HCHK(c, launch_to_double(c->den32, c->den, c->n, c->st));
         if ((rc = deriv_dev(c, 1, which - HETD_F_DXDEN + 1, 0, c->den, c->tmp))) return rc;
         src = c->tmp;
         break;
@@ -845,7 +917,8 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
     case HETD_F_PENALTY: {
         if (!c->p.lsolid) return fail(c, HETD_ERR_ARG, "no penalty without lsolid");
         std::vector<double> d(c->n);
-        HCHK(c, hipMemcpyAsync(d.data(), c->den, nR, hipMemcpyDeviceToHost, c->st));
+        if (mx) HCHK(c, launch_to_double(c->den32, c->tmp, c->n, c->st));
+        HCHK(c, hipMemcpyAsync(d.data(), mx ? c->tmp : c->den, nR, hipMemcpyDeviceToHost, c->st));
         HCHK(c, hipStreamSynchronize(c->st));
         for (long i = 0; i < c->n; ++i) {
             const double t = std::tanh(c->p.beta_solid * (d[i] - c->p.den_m));
@@ -861,6 +934,7 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
     default:
         return fail(c, HETD_ERR_ARG, "unknown field %d", which);
     }
+    if (!src) return fail(c, HETD_ERR_HIP, "conversion launch failed");
     HCHK(c, hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, c->st));
     HCHK(c, hipStreamSynchronize(c->st));
     return HETD_OK;

@@ -46,6 +46,11 @@ template <typename T> struct ZPassArgs {
 template <typename T> hipError_t launch_xpass(XPassArgs<T> a, hipStream_t st);
 template <typename T> hipError_t launch_ypass(YPassArgs<T> a, int nz, int nbuf, hipStream_t st);
 template <typename T> hipError_t launch_zpass(ZPassArgs<T> a, int nbuf, hipStream_t st);
+// register-resident passes for axis lengths 16, 64, 256 (fft_reg.hip)
+bool reg_fft_supported(int n);
+template <typename T> hipError_t launch_xreg(XPassArgs<T> a, hipStream_t st);
+template <typename T> hipError_t launch_yreg(YPassArgs<T> a, int nz, int nbuf, hipStream_t st);
+template <typename T> hipError_t launch_zreg(ZPassArgs<T> a, int nbuf, hipStream_t st);
 
 // --------------------------------------------------------------- grid info
 
@@ -98,6 +103,8 @@ struct UpdateArgs {
     double2* pc;
     double* partial;  // CORR: per-block sum |pc|
     int* guard;       // RK: set if den not > 0
+    const float* pot4f;  // mixed precision: FP32 mean field (else pot4)
+    float* denf;         // mixed precision: FP32 density output (else den)
 };
 
 hipError_t launch_update(const UpdateArgs& a, hipStream_t st, int* nblocks_out);
@@ -120,7 +127,8 @@ struct ForceArgs {
     StencilCoef sc;
     const double* den;
     const ImpState* imp;
-    double* partial;  // 3 per block
+    double* partial;    // 3 per block
+    const float* denf;  // mixed precision: FP32 density, FP32 stencil
 };
 hipError_t launch_force_partial(const ForceArgs& a, hipStream_t st, int* nblocks);
 
@@ -143,8 +151,16 @@ struct EnergyArgs {
     const double* inty;
     const double* intz;
     double* partial;  // EN_NCOMP per block
+    const float* dencgf;  // mixed precision fields (else dencg / delj4)
+    const float* delj4f;
 };
 hipError_t launch_energy_partial(const EnergyArgs& a, hipStream_t st, int* nblocks);
+
+// LDS-tiled versions (npd 9 and 13); one partial per tile
+bool tiled_supported(const StencilCoef& sc);
+hipError_t launch_update_tiled(const UpdateArgs& a, hipStream_t st, int* nblocks);
+hipError_t launch_force_tiled(const ForceArgs& a, hipStream_t st, int* nblocks);
+hipError_t launch_energy_tiled(const EnergyArgs& a, hipStream_t st, int* nblocks);
 
 // out[c] = sum_b partial[b*ncomp + c], fixed order (bitwise reproducible)
 hipError_t launch_reduce_final(const double* partial, int nblocks, int ncomp, double* out, hipStream_t st);
@@ -165,6 +181,9 @@ hipError_t launch_traj_record(const ImpState* imp, double* rec, hipStream_t st);
 // ---------------------------------------------------------- pointwise
 
 hipError_t launch_den_from_psi(const double2* psi, double* den, long n, hipStream_t st);
+hipError_t launch_den32_from_psi(const double2* psi, float* den, long n, hipStream_t st);
+hipError_t launch_to_double(const float* a, double* out, long n, hipStream_t st);
+hipError_t launch_to_float(const double* a, float* out, long n, hipStream_t st);
 hipError_t launch_uimp_field(const Grid& g, const ImpState* imp, double* out, hipStream_t st);
 // OTC helpers (term_alfa.f90)
 hipError_t launch_falfs(const double* denalf, double den0s, double* falfs, long n, hipStream_t st);

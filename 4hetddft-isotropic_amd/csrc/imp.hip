@@ -201,6 +201,36 @@ hipError_t launch_den_from_psi(const double2* psi, double* den, long n, hipStrea
     return hipGetLastError();
 }
 
+__global__ void k_den32_from_psi(const double2* psi, float* den, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double a = hypot(psi[i].x, psi[i].y);
+        den[i] = (float)(a * a);
+    }
+}
+hipError_t launch_den32_from_psi(const double2* psi, float* den, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_den32_from_psi, grid1(n), dim3(256), 0, st, psi, den, n);
+    return hipGetLastError();
+}
+
+__global__ void k_to_double(const float* a, double* out, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (double)a[i];
+}
+hipError_t launch_to_double(const float* a, double* out, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_to_double, grid1(n), dim3(256), 0, st, a, out, n);
+    return hipGetLastError();
+}
+
+__global__ void k_to_float(const double* a, float* out, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)a[i];
+}
+hipError_t launch_to_float(const double* a, float* out, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_to_float, grid1(n), dim3(256), 0, st, a, out, n);
+    return hipGetLastError();
+}
+
 __global__ void k_uimp_field(Grid g, const ImpState* imp, double* out) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.n) return;

@@ -1,0 +1,356 @@
+// stencil_tiled.hip -- LDS-tiled 2.5-D stencil engine for gfx950.
+//
+// A workgroup owns a 64 (x) x 8 (y) column of the grid and marches it through
+// a chunk of z planes.  Per plane the 64x8 tile plus an M-point halo in x and
+// y sits in LDS (x and y stencils read LDS); each thread keeps the z column of
+// its two points in a register queue of 2M+1 values (the z stencil reads
+// registers), so every grid value is fetched from HBM about once.  Tiles are
+// ordered so that consecutive tiles, which share halos, run on the same XCD
+// (blocks are dealt to the 8 XCDs round-robin).
+//
+// Three consumers of the stencils share the engine:
+//   UPDATE: Laplacian of psi (npd-point second derivative, steprk.f90:67-69,
+//           steppc.f90:45-47) + right-hand side + RK-Gill stage / predictor /
+//           corrector update;
+//   FORCE:  gradient of den (derden.f90:26-28) times u_imp, summed
+//           (potenimp.f90:44-46);
+//   ENERGY: gradient of psi and the energy densities (energy.f90:51-110).
+#include "hetd_kernels.h"
+
+namespace hetd {
+
+constexpr int TTX = 64, TTY = 8, TNT = 256, TPY = 2;
+
+template <typename V> struct vz;
+template <> struct vz<float> {
+    static __device__ __forceinline__ float zero() { return 0.0f; }
+    static __device__ __forceinline__ float scale(float v, double s) { return v * (float)s; }
+    static __device__ __forceinline__ void fma_(float& s, double c, float v) { s = fmaf((float)c, v, s); }
+};
+template <> struct vz<double> {
+    static __device__ __forceinline__ double zero() { return 0.0; }
+    static __device__ __forceinline__ double scale(double v, double s) { return v * s; }
+    static __device__ __forceinline__ void fma_(double& s, double c, double v) { s = fma(c, v, s); }
+};
+template <> struct vz<double2> {
+    static __device__ __forceinline__ double2 zero() { return make_double2(0.0, 0.0); }
+    static __device__ __forceinline__ double2 scale(double2 v, double s) { return make_double2(v.x * s, v.y * s); }
+    static __device__ __forceinline__ void fma_(double2& s, double c, double2 v) {
+        s.x = fma(c, v.x, s.x);
+        s.y = fma(c, v.y, s.y);
+    }
+};
+
+// value of f at (ix, iy, iz) where at most one coordinate may be outside the
+// box: zero (icon 13) or wrapped (icon 8; icon 18 flips the sign)
+template <typename V>
+__device__ __forceinline__ V load_bc(const V* __restrict__ f, int nx, int ny, int nz, int ix, int iy, int iz,
+                                     int icon) {
+    double sg = 1.0;
+    if (ix < 0 || ix >= nx) {
+        if (icon == 13) return vz<V>::zero();
+        ix = ix < 0 ? ix + nx : ix - nx;
+        if (icon == 18) sg = -sg;
+    }
+    if (iy < 0 || iy >= ny) {
+        if (icon == 13) return vz<V>::zero();
+        iy = iy < 0 ? iy + ny : iy - ny;
+        if (icon == 18) sg = -sg;
+    }
+    if (iz < 0 || iz >= nz) {
+        if (icon == 13) return vz<V>::zero();
+        iz = iz < 0 ? iz + nz : iz - nz;
+        if (icon == 18) sg = -sg;
+    }
+    const V v = f[(long)ix + (long)nx * ((long)iy + (long)ny * iz)];
+    return sg == 1.0 ? v : vz<V>::scale(v, sg);
+}
+
+enum TKind { TK_UPDATE = 0, TK_FORCE = 1, TK_ENERGY = 2, TK_FORCE32 = 3 };
+
+template <int KIND> struct TArgs;
+template <> struct TArgs<TK_UPDATE> { using V = double2; using A = UpdateArgs; static constexpr int NC = 1; };
+template <> struct TArgs<TK_FORCE> { using V = double; using A = ForceArgs; static constexpr int NC = 3; };
+template <> struct TArgs<TK_ENERGY> { using V = double2; using A = EnergyArgs; static constexpr int NC = EN_NCOMP; };
+template <> struct TArgs<TK_FORCE32> { using V = float; using A = ForceArgs; static constexpr int NC = 3; };
+
+struct TiledCommon {
+    Grid g;
+    StencilCoef sc;
+    int zc;          // z planes per workgroup
+    int ntx, nty, ntz;
+};
+
+// value of f at grid column (gx, gy), plane iz (in range), for an LDS cell:
+// the boundary rule when one of gx, gy is outside the box, zero for cells no
+// valid point reads (corners, beyond one halo width, axes shorter than the
+// stencil, where the reference's derivative is zero)
+template <typename V, int M>
+__device__ __forceinline__ V cell_value(const V* __restrict__ f, int nx, int ny, int nz, int gx, int gy, int iz,
+                                        int icon) {
+    constexpr int W = 2 * M + 1;
+    const bool ox = gx < 0 || gx >= nx, oy = gy < 0 || gy >= ny;
+    if (ox && oy) return vz<V>::zero();
+    if (ox && (nx < W || gx < -M || gx >= nx + M)) return vz<V>::zero();
+    if (oy && (ny < W || gy < -M || gy >= ny + M)) return vz<V>::zero();
+    return load_bc<V>(f, nx, ny, nz, gx, gy, iz, icon);
+}
+
+template <int KIND, int M>
+__global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KIND>::A args) {
+    using V = typename TArgs<KIND>::V;
+    constexpr int NC = TArgs<KIND>::NC;
+    constexpr int LX = TTX + 2 * M, LY = TTY + 2 * M, W = 2 * M + 1;
+    __shared__ V lds[LX * LY];
+    __shared__ double red[NC * TNT / 64];
+
+    const Grid& g = tc.g;
+    const int nx = g.nx, ny = g.ny, nz = g.nz, icon = tc.sc.icon;
+    // XCD-aware tile order: the grid is padded to 8*per blocks and the blocks
+    // dealt to one XCD (b % 8 equal) take a contiguous range of per tiles
+    const int nb = tc.ntx * tc.nty * tc.ntz;
+    const int b = blockIdx.x;
+    const int per = (nb + 7) / 8;
+    const int t = (b % 8) * per + b / 8;
+    if (t >= nb) return;  // padding block: the whole workgroup leaves
+    const int tile_x = t % tc.ntx, tile_y = (t / tc.ntx) % tc.nty, tile_z = t / (tc.ntx * tc.nty);
+    const int ix0 = tile_x * TTX, iy0 = tile_y * TTY, iz0 = tile_z * tc.zc;
+    const int izend = min(iz0 + tc.zc, nz);
+    const int tx = threadIdx.x % TTX, ty = threadIdx.x / TTX;  // ty 0..3
+    const int ix = ix0 + tx;
+    const V* __restrict__ f;
+    const double* c1;
+    const double* c2;
+    const double* c3;
+    if constexpr (KIND == TK_UPDATE) {
+        f = args.psi;
+        c1 = tc.sc.c2[0]; c2 = tc.sc.c2[1]; c3 = tc.sc.c2[2];
+    } else if constexpr (KIND == TK_FORCE) {
+        f = args.den;
+        c1 = tc.sc.c1[0]; c2 = tc.sc.c1[1]; c3 = tc.sc.c1[2];
+    } else if constexpr (KIND == TK_FORCE32) {
+        f = args.denf;
+        c1 = tc.sc.c1[0]; c2 = tc.sc.c1[1]; c3 = tc.sc.c1[2];
+    } else {
+        f = args.psi;
+        c1 = tc.sc.c1[0]; c2 = tc.sc.c1[1]; c3 = tc.sc.c1[2];
+    }
+    const bool zok = nz >= W;  // one periodic wrap reaches every z neighbour
+    double acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+
+    int iy[TPY];
+    bool valid[TPY];
+    V q[TPY][W];
+#pragma unroll
+    for (int p = 0; p < TPY; ++p) {
+        iy[p] = iy0 + ty + 4 * p;
+        valid[p] = ix < nx && iy[p] < ny;
+#pragma unroll
+        for (int k = 0; k < W - 1; ++k) {
+            const int zz = iz0 - M + k;
+            q[p][k] = (valid[p] && (zok || (zz >= 0 && zz < nz))) ? load_bc<V>(f, nx, ny, nz, ix, iy[p], zz, icon)
+                                                                   : vz<V>::zero();
+        }
+    }
+
+    for (int iz = iz0; iz < izend; ++iz) {
+#pragma unroll
+        for (int p = 0; p < TPY; ++p) {
+            const int zz = iz + M;
+            q[p][W - 1] = (valid[p] && (zok || zz < nz)) ? load_bc<V>(f, nx, ny, nz, ix, iy[p], zz, icon)
+                                                          : vz<V>::zero();
+        }
+        __syncthreads();
+        // plane iz into LDS: interior from the queues, halo from memory
+#pragma unroll
+        for (int p = 0; p < TPY; ++p)
+            lds[(ty + 4 * p + M) * LX + tx + M] =
+                valid[p] ? q[p][M] : cell_value<V, M>(f, nx, ny, nz, ix, iy[p], iz, icon);
+        constexpr int NHX = 2 * M * TTY, NH = NHX + 2 * M * TTX;
+        for (int h = threadIdx.x; h < NH; h += TNT) {
+            int lx, ly;
+            if (h < NHX) {  // x halo: M columns on each side of the tile rows
+                const int r = h / (2 * M), cc = h % (2 * M);
+                ly = r + M;
+                lx = cc < M ? cc : TTX + cc;
+            } else {        // y halo: M rows above and below the tile columns
+                const int hh = h - NHX;
+                const int r = hh / TTX, cc = hh % TTX;
+                lx = cc + M;
+                ly = r < M ? r : TTY + r;
+            }
+            const int gx = ix0 + lx - M, gy = iy0 + ly - M;
+            lds[ly * LX + lx] = cell_value<V, M>(f, nx, ny, nz, gx, gy, iz, icon);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < TPY; ++p) {
+            if (!valid[p]) continue;
+            const int ly = ty + 4 * p + M;
+            V sx = vz<V>::zero(), sy = vz<V>::zero(), sz = vz<V>::zero();
+            if (nx >= 2 * M + 1)
+#pragma unroll
+                for (int j = 0; j < W; ++j) vz<V>::fma_(sx, c1[j], lds[ly * LX + tx + j]);
+            if (ny >= 2 * M + 1)
+#pragma unroll
+                for (int j = 0; j < W; ++j) vz<V>::fma_(sy, c2[j], lds[(ly - M + j) * LX + tx + M]);
+            if (nz >= 2 * M + 1)
+#pragma unroll
+                for (int j = 0; j < W; ++j) vz<V>::fma_(sz, c3[j], q[p][j]);
+            const long o = (long)ix + (long)nx * ((long)iy[p] + (long)ny * iz);
+            const double* R = args.imp->rpot;
+            const double dz = g.z[iz] - R[2], dy = g.y[iy[p]] - R[1];
+            const double u = uimp_at(g.potion, g.npot, g.delinter, g.x[ix] - R[0], dy * dy + dz * dz);
+            if constexpr (KIND == TK_UPDATE) {
+                const UpdateArgs& a = args;
+                const double lr = (sx.x + sy.x) + sz.x, li = (sx.y + sy.y) + sz.y;
+                const double2 ps = q[p][M];
+                const double pot = a.pot4f ? (double)a.pot4f[o] : a.pot4[o];
+                const double tr = g.lambdah * ((((((1.0 + g.tcx[ix]) + 1.0) + g.tcy[iy[p]]) + 1.0) + g.tcz[iz]));
+                const double ar = lr * a.h2o2m4 - pot * ps.x;
+                const double ai = li * a.h2o2m4 - pot * ps.y;
+                double2 fv;
+                fv.x = (tr * ar - ai) + u * ps.y;
+                fv.y = (tr * ai + ar) - u * ps.x;
+                double2 pn;
+                if (a.kind == STAGE_RK) {
+                    const double2 qq = a.q[o];
+                    double2 s1, qn;
+                    s1.x = a.ra * (fv.x - a.rb * qq.x);
+                    s1.y = a.ra * (fv.y - a.rb * qq.y);
+                    qn.x = (qq.x + 3.0 * s1.x) - a.rc * fv.x;
+                    qn.y = (qq.y + 3.0 * s1.y) - a.rc * fv.y;
+                    a.q[o] = qn;
+                    if (a.rk_j == 0) a.hnew[o] = fv;
+                    pn.x = ps.x + a.dt * s1.x;
+                    pn.y = ps.y + a.dt * s1.y;
+                    const double h = hypot(pn.x, pn.y), d = h * h;
+                    a.psi_out[o] = pn;
+                    if (a.denf) a.denf[o] = (float)d;
+                    else a.den[o] = d;
+                    if (!(d > 0.0)) atomicOr(a.guard, 1);
+                } else if (a.kind == STAGE_PRED) {
+                    const double c4 = (4.0 / 3.0) * a.dt;
+                    const double2 p3 = a.po3[o], h1 = a.h1[o], h2 = a.h2[o], pc = a.pc[o];
+                    double2 P;
+                    P.x = p3.x + c4 * ((2.0 * fv.x - h1.x) + 2.0 * h2.x);
+                    P.y = p3.y + c4 * ((2.0 * fv.y - h1.y) + 2.0 * h2.y);
+                    const double c112 = 112.0 / 121.0;
+                    pn.x = P.x - c112 * pc.x;
+                    pn.y = P.y - c112 * pc.y;
+                    a.psi_out[o] = pn;
+                    a.pc[o] = P;
+                    a.hnew[o] = fv;
+                    const double h = hypot(pn.x, pn.y);
+                    if (a.denf) a.denf[o] = (float)(h * h);
+                    else a.den[o] = h * h;
+                } else {
+                    const double2 p3 = a.po3[o], p2 = a.po2[o], h1 = a.h1[o], h2 = a.h2[o], pc = a.pc[o];
+                    const double t3 = 3.0 * a.dt;
+                    double2 Cc, pcn;
+                    Cc.x = 0.125 * ((9.0 * p3.x - p2.x) + t3 * ((fv.x + 2.0 * h1.x) - h2.x));
+                    Cc.y = 0.125 * ((9.0 * p3.y - p2.y) + t3 * ((fv.y + 2.0 * h1.y) - h2.y));
+                    pcn.x = pc.x - Cc.x;
+                    pcn.y = pc.y - Cc.y;
+                    const double c9 = 9.0 / 121.0;
+                    pn.x = Cc.x + c9 * pcn.x;
+                    pn.y = Cc.y + c9 * pcn.y;
+                    a.pc[o] = pcn;
+                    a.psi_out[o] = pn;
+                    const double h = hypot(pn.x, pn.y);
+                    if (a.denf) a.denf[o] = (float)(h * h);
+                    else a.den[o] = h * h;
+                    acc[0] += hypot(pcn.x, pcn.y);
+                }
+            } else if constexpr (KIND == TK_FORCE || KIND == TK_FORCE32) {
+                acc[0] += (double)sx * u;
+                acc[1] += (double)sy * u;
+                acc[2] += (double)sz * u;
+            } else {
+                const EnergyArgs& a = args;
+                const double h1 = hypot(sx.x, sx.y), h2 = hypot(sy.x, sy.y), h3 = hypot(sz.x, sz.y);
+                acc[EN_KIN] += (h1 * h1 + h2 * h2) + h3 * h3;
+                // den = |psi|^2 of the FP64 state (equal to the den array in FP64)
+                const double hp = hypot(q[p][M].x, q[p][M].y), d = hp * hp;
+                const double cg = a.dencgf ? (double)a.dencgf[o] : a.dencg[o];
+                acc[EN_LJ] += (a.delj4f ? (double)a.delj4f[o] : a.delj4[o]) * d;
+                acc[EN_COR] += d * cg * cg * (a.aux1 + a.aux2 * cg);
+                if (a.lsolid) acc[EN_SOLID] += d * (1.0 + tanh(a.beta * (d - a.den_m)));
+                acc[EN_HEX] += u * d;
+                if (a.otc)
+                    acc[EN_ALPHA] += a.falfs[o] * ((a.dxden[o] * a.intx[o] + a.dyden[o] * a.inty[o]) +
+                                                   a.dzden[o] * a.intz[o]);
+                acc[EN_NORM] += d;
+            }
+        }
+        // shift the z queues
+#pragma unroll
+        for (int p = 0; p < TPY; ++p)
+#pragma unroll
+            for (int k = 0; k < W - 1; ++k) q[p][k] = q[p][k + 1];
+    }
+    bool reduce = true;
+    if constexpr (KIND == TK_UPDATE) reduce = args.kind == STAGE_CORR;
+    if (reduce) {
+        block_sum<NC>(acc, red);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) args.partial[(long)t * NC + c] = acc[c];
+        }
+    }
+}
+
+// ------------------------------------------------------------- launchers
+
+static TiledCommon tiled_common(const Grid& g, const StencilCoef& sc) {
+    TiledCommon tc;
+    tc.g = g;
+    tc.sc = sc;
+    tc.ntx = (g.nx + TTX - 1) / TTX;
+    tc.nty = (g.ny + TTY - 1) / TTY;
+    const long nbxy = (long)tc.ntx * tc.nty;
+    // aim at >= ~1024 workgroups (256 CUs), z chunks of 4..64 planes
+    long zc = (g.nz * nbxy + 1023) / 1024;
+    if (zc < 4) zc = 4;
+    if (zc > 64) zc = 64;
+    if (zc > g.nz) zc = g.nz;
+    const int nchunk = (int)((g.nz + zc - 1) / zc);
+    tc.zc = (g.nz + nchunk - 1) / nchunk;
+    tc.ntz = (g.nz + tc.zc - 1) / tc.zc;
+    return tc;
+}
+
+int tiled_blocks(const Grid& g, const StencilCoef& sc) {
+    TiledCommon tc = tiled_common(g, sc);
+    return tc.ntx * tc.nty * tc.ntz;
+}
+
+bool tiled_supported(const StencilCoef& sc) { return sc.m == 4 || sc.m == 6; }
+
+template <int KIND>
+static hipError_t launch_tiled(const typename TArgs<KIND>::A& a, hipStream_t st, int* nblocks) {
+    TiledCommon tc = tiled_common(a.g, a.sc);
+    const int nb = tc.ntx * tc.nty * tc.ntz;
+    const int nlaunch = 8 * ((nb + 7) / 8);
+    if (nblocks) *nblocks = nb;
+    switch (a.sc.m) {
+    case 4: hipLaunchKernelGGL((k_tiled<KIND, 4>), dim3(nlaunch), dim3(TNT), 0, st, tc, a); break;
+    case 6: hipLaunchKernelGGL((k_tiled<KIND, 6>), dim3(nlaunch), dim3(TNT), 0, st, tc, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_update_tiled(const UpdateArgs& a, hipStream_t st, int* nblocks) {
+    return launch_tiled<TK_UPDATE>(a, st, nblocks);
+}
+hipError_t launch_force_tiled(const ForceArgs& a, hipStream_t st, int* nblocks) {
+    return a.denf ? launch_tiled<TK_FORCE32>(a, st, nblocks) : launch_tiled<TK_FORCE>(a, st, nblocks);
+}
+hipError_t launch_energy_tiled(const EnergyArgs& a, hipStream_t st, int* nblocks) {
+    return launch_tiled<TK_ENERGY>(a, st, nblocks);
+}
+
+}  // namespace hetd

@@ -116,44 +116,20 @@ def cpu_baseline(n, nthreads, steps=2):
             "sample": "1 predictor-corrector step of He-1000+Xe %d^3 FP64, C oracle, %.1f s" % (m, sec)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--kernel", default="update", choices=["update", "xpass", "ypass", "zpass", "force"])
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the ensemble split across "
-              "GPUs is described in DESIGN.md)" % args.gpus, file=sys.stderr)
-        sys.exit(2)
-    warmup = max(args.warmup, 3)
-
-    import numpy as np
+def run_precision(args, prec, psi, kclass, ev):
+    """Warm up, then time args.steps PC iterations; returns a result dict."""
     from hetd import Context, capi
-    from hetd.initstate import woods_saxon_state
-
     n = args.n
     xm = 0.2 * n
-    p = capi.default_params(nx=n, ny=n, nz=n, xmax=xm, ymax=xm, zmax=xm, npd=13, icon=13)
+    p = capi.default_params(nx=n, ny=n, nz=n, xmax=xm, ymax=xm, zmax=xm, npd=13, icon=13, precision=prec)
     t0 = time.perf_counter()
-    psi = woods_saxon_state(n, n, n, xm, xm, xm, 1000, 3)
     ctx = Context(p, selec="Xe_He", r_cutoff=2.55, umax=2793.9365463)
     ctx.set_state(psi, (0.0, 0.0, 0.0), (0.0, 0.0, 2.0 * KTOPS))
-    del psi
     ctx.init()
     dt = 5e-4 / KTOPS
-    ctx.evolve(dt, warmup, traj=False)
+    ctx.evolve(dt, max(args.warmup, 3), traj=False)
     e0 = ctx.energy()
     setup_s = time.perf_counter() - t0
-    print("setup %.1f s, E/N %.6f K" % (setup_s, e0["etot4"] / 1000.0), file=sys.stderr, flush=True)
-
-    kclass = {"update": capi.K_UPDATE, "xpass": capi.K_XPASS, "ypass": capi.K_YPASS,
-              "zpass": capi.K_ZPASS, "force": capi.K_FORCE}[args.kernel]
-    ev = HipEvents()
     stream = ctx.stream()
     a, b = ev.event(), ev.event()
     ctx.timing(kclass)
@@ -165,8 +141,6 @@ def main():
     wall = time.perf_counter() - w0
     kms, klaunch, kbytes = ctx.timing_read()
     ctx.timing(capi.K_OFF)
-
-    # per-class breakdown (short extra runs, one class at a time)
     classes = {}
     for kc, name in capi.K_NAMES.items():
         ctx.timing(kc)
@@ -176,16 +150,69 @@ def main():
                          "GBps": round(by / (t * 1e-3) / 1e9, 1) if t > 0 else None}
     ctx.timing(capi.K_OFF)
     e1 = ctx.energy()
+    norm = ctx.norm()
     ctx.close()
+    nsteps_total = max(args.warmup, 3) + args.steps + 2 * len(capi.K_NAMES)
+    return {"ms": ms, "wall": wall, "kms": kms, "klaunch": klaunch, "kbytes": kbytes, "classes": classes,
+            "e0": e0, "e1": e1, "norm": norm, "setup_s": setup_s, "nsteps_total": nsteps_total}
 
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"])
+    ap.add_argument("--kernel", default="update", choices=["update", "xpass", "ypass", "zpass", "force"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the ensemble split across "
+              "GPUs is described in DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    from hetd import capi
+    from hetd.initstate import woods_saxon_state
+
+    n = args.n
+    xm = 0.2 * n
+    psi = woods_saxon_state(n, n, n, xm, xm, xm, 1000, 3)
+    kclass = {"update": capi.K_UPDATE, "xpass": capi.K_XPASS, "ypass": capi.K_YPASS,
+              "zpass": capi.K_ZPASS, "force": capi.K_FORCE}[args.kernel]
+    ev = HipEvents()
+    res = {}
+    if args.precision in ("both", "fp64"):
+        res["fp64"] = run_precision(args, capi.HETD_FP64, psi, kclass, ev)
+    if args.precision in ("both", "mixed"):
+        res["mixed"] = run_precision(args, capi.HETD_MIXED, psi, kclass, ev)
+    del psi
+    head = "mixed" if "mixed" in res else "fp64"
+    r = res[head]
     pts = float(n) ** 3
-    ms_step = ms / args.steps
-    value = pts * args.steps / (ms * 1e-3) / 1e6
-    avg_launch_s = kms * 1e-3 / max(klaunch, 1)
-    achieved = (kbytes / max(klaunch, 1)) / avg_launch_s / 1e9
-    step_bytes = model_bytes_per_point_step(n) * pts
+    ms_step = r["ms"] / args.steps
+    value = pts * args.steps / (r["ms"] * 1e-3) / 1e6
+    avg_launch_s = r["kms"] * 1e-3 / max(r["klaunch"], 1)
+    achieved = (r["kbytes"] / max(r["klaunch"], 1)) / avg_launch_s / 1e9
+    per_prec = {}
+    for k, v in res.items():
+        mstep = v["ms"] / args.steps
+        per_prec[k] = {"Mpoint_steps_per_s": round(pts / (mstep * 1e-3) / 1e6, 2), "ms_per_step": round(mstep, 4),
+                       "step_hbm_roof_frac": round(model_bytes_per_point_step(n, k == "mixed") * pts /
+                                                   (mstep * 1e-3) / HBM_PEAK, 4),
+                       "model_bytes_per_point_step": round(model_bytes_per_point_step(n, k == "mixed"), 2),
+                       "E_per_atom_end": v["e1"]["etot4"] / 1000.0, "norm_end": v["norm"],
+                       "kernel_classes": v["classes"]}
+    cmp_ = {}
+    if len(res) == 2:
+        a64, amx = res["fp64"], res["mixed"]
+        cmp_ = {"speedup_mixed_over_fp64": round(a64["ms"] / amx["ms"], 3),
+                "dE_over_E_mixed_vs_fp64_etot4": abs(amx["e1"]["etot4"] - a64["e1"]["etot4"]) / abs(a64["e1"]["etot4"]),
+                "dE_over_E_mixed_vs_fp64_etot": abs(amx["e1"]["etot"] - a64["e1"]["etot"]) / abs(a64["e1"]["etot"]),
+                "dnorm_over_norm": abs(amx["norm"] - a64["norm"]) / a64["norm"],
+                "steps_compared": a64["nsteps_total"]}
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_%s_fp64_%d.json" % (args.kernel, n))
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s_%d.json" % (args.kernel, head, n))
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -199,24 +226,20 @@ def main():
         "unit": "Mpoint-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": warmup,
+        "warmup": max(args.warmup, 3),
         "ms_per_step": round(ms_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "mixed: f32 density/mean field/FFT, f64 state and reductions" if head == "mixed" else "f64",
         "data": "synthetic (Woods-Saxon He-1000 droplet, seeded phase, Xe at the centre moving 200 m/s)",
-        "config": {"workload": "He-1000+Xe %d^3 0.4A OT+solid npd13 icon13 dt0.5fs FP64 predictor-corrector "
-                               "steps" % n, "grid": [n, n, n], "precision": "fp64",
-                   "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
-                   "step_hbm_roof_frac": round(step_bytes / (ms_step * 1e-3) / HBM_PEAK, 4),
-                   "model_bytes_per_point_step": round(model_bytes_per_point_step(n), 2),
-                   "dE_over_E_during_timed": abs(e1["etot"] - e0["etot"]) / abs(e0["etot"]),
-                   "kernel_classes": classes},
+        "config": {"workload": "He-1000+Xe %d^3 0.4A OT+solid npd13 icon13 dt0.5fs predictor-corrector steps" % n,
+                   "grid": [n, n, n], "precision": head, "wall_ms_per_step": round(r["wall"] * 1e3 / args.steps, 4),
+                   "per_precision": per_prec, "mixed_vs_fp64": cmp_},
         "roofline": {"bound": "hbm", "kernel": args.kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4),
-                     "traffic": traffic, "launches": klaunch,
-                     "bytes_per_launch": kbytes / max(klaunch, 1),
+                     "traffic": traffic, "launches": r["klaunch"],
+                     "bytes_per_launch": r["kbytes"] / max(r["klaunch"], 1),
                      "avg_launch_us": round(avg_launch_s * 1e6, 2)},
         "cpu_baseline": cpu,
     }

@@ -26,7 +26,8 @@ def make_pair(oracle_mod, p, selec="null", r_cutoff=2.0, umax=0.0):
 # --------------------------------------------------------------------- FFT
 
 @pytest.mark.parametrize("shape", [(8, 6, 10), (27, 10, 12), (16, 16, 16), (5, 9, 45), (20, 12, 36),
-                                   (32, 24, 64), (40, 30, 50)])
+                                   (32, 24, 64), (40, 30, 50), (16, 64, 256), (256, 16, 64), (64, 256, 16),
+                                   (15, 9, 256)])
 def test_fft_matches_numpy_and_oracle(oracle_mod, shape):
     say("fft", shape)
     nz, ny, nx = shape
@@ -159,7 +160,8 @@ def test_fine_grained_calls_match_oracle(oracle_mod):
 
 
 @pytest.mark.parametrize("shape,npd,icon,core", [((48, 40, 36), 13, 13, "OT"), ((30, 32, 45), 9, 8, "OT"),
-                                                 ((36, 36, 36), 13, 18, "OTC")])
+                                                 ((36, 36, 36), 13, 18, "OTC"), ((64, 64, 64), 13, 13, "OT"),
+                                                 ((16, 64, 64), 9, 8, "OTC")])
 def test_evolve_matches_oracle_other_grids(oracle_mod, shape, npd, icon, core):
     say("oracle evolve", shape, npd, icon, core)
     from hetd.initstate import woods_saxon_state
@@ -191,3 +193,36 @@ def test_guard_reports_nonpositive_density():
         ctx.init()
         with pytest.raises(RuntimeError, match="den not > 0"):
             ctx.steprk(G.DT)
+
+
+# ------------------------------------------------------ mixed precision
+
+@pytest.mark.parametrize("shape", [(64, 64, 64), (36, 40, 48)])
+def test_mixed_precision_tracks_fp64(oracle_mod, shape):
+    """HETD_MIXED: FP32 density, mean field and FFTs, FP64 integrator state,
+    reductions and impurity.  Reported against the FP64 path (which matches
+    the oracle to 1e-10): relative energy error <= 1e-6, psi within 1e-5."""
+    say("mixed", shape)
+    from hetd.initstate import woods_saxon_state
+    nz, ny, nx = shape
+    h = 0.6
+    kw = dict(nx=nx, ny=ny, nz=nz, xmax=nx * h / 2, ymax=ny * h / 2, zmax=nz * h / 2, npd=13, icon=13)
+    psi = woods_saxon_state(nx, ny, nz, kw["xmax"], kw["ymax"], kw["zmax"], 150, 3)
+    out = {}
+    for prec in (capi.HETD_FP64, capi.HETD_MIXED):
+        p = capi.default_params(precision=prec, **kw)
+        with Context(p, selec="Xe_He", r_cutoff=2.55, umax=2793.9365463) as ctx:
+            ctx.set_state(psi, (0.3, -0.2, 0.1), (1.0, 2.0, -3.0))
+            ctx.init()
+            t = ctx.evolve(G.DT, 12)
+            out[prec] = (t, ctx.field(capi.F_PSI), ctx.energy(), ctx.norm(), ctx.field(capi.F_POT4))
+    t64, p64, e64, n64, u64 = out[capi.HETD_FP64]
+    t32, p32, e32, n32, u32 = out[capi.HETD_MIXED]
+    say("mixed errors: psi %.2e E %.2e norm %.2e pot4 %.2e" % (
+        G.rel(p32, p64), abs(e32["etot4"] - e64["etot4"]) / abs(e64["etot4"]), abs(n32 - n64) / n64,
+        G.rel(u32, u64)))
+    assert G.rel(p32, p64) < 1e-5
+    assert abs(e32["etot4"] - e64["etot4"]) <= 1e-6 * abs(e64["etot4"])
+    assert abs(e32["etot"] - e64["etot"]) <= 1e-6 * abs(e64["etot"])
+    assert abs(n32 - n64) <= 1e-9 * n64
+    assert np.abs(t32[:, 2:5] - t64[:, 2:5]).max() < 1e-8 * max(np.abs(t64[:, 2:5]).max(), 1.0)
