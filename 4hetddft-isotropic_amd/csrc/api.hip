@@ -160,7 +160,7 @@ template <typename T> static hipError_t run_z(hetd_ctx* c, ZPassArgs<T> z, int n
 static hipError_t run_update(hetd_ctx* c, const UpdateArgs& u, int* nb) {
     double per = 144.0;  // PRED / CORR (FP64 den and pot4)
     if (u.kind == STAGE_RK) per = u.rk_j == 0 ? 96.0 : 80.0;
-    if (u.denf) per -= 8.0;  // FP32 den write and pot4 read
+    // mixed: the FP32 den copy written (+4) balances the FP32 pot4 read (-4)
     tbeg(c, HETD_K_UPDATE);
     hipError_t e = tiled_supported(c->sc) ? launch_update_tiled(u, c->st, nb) : launch_update(u, c->st, nb);
     tend(c, HETD_K_UPDATE, per * c->n);
@@ -170,7 +170,7 @@ static hipError_t run_update(hetd_ctx* c, const UpdateArgs& u, int* nb) {
 static hipError_t run_force(hetd_ctx* c, const ForceArgs& f, int* nb) {
     tbeg(c, HETD_K_FORCE);
     hipError_t e = tiled_supported(c->sc) ? launch_force_tiled(f, c->st, nb) : launch_force_partial(f, c->st, nb);
-    tend(c, HETD_K_FORCE, (f.denf ? 4.0 : 8.0) * c->n);
+    tend(c, HETD_K_FORCE, 8.0 * c->n);
     return e;
 }
 
@@ -358,8 +358,8 @@ static int poten_dev(hetd_ctx* c, bool want_ccor, bool want_fden) {
 // forceimp + aimp = F/mAg_u on the device (potenimp.f90:24-55, DFT4He3d.f90:852)
 static int force_dev(hetd_ctx* c) {
     ForceArgs f{};
+    // FP64 den in both modes: the force is a sum with heavy cancellation
     f.g = c->grid; f.sc = c->sc; f.den = c->den; f.imp = c->imp; f.partial = c->partial;
-    if (c->mixed) f.denf = c->den32;
     int nb = 0;
     HCHK(c, run_force(c, f, &nb));
     HCHK(c, launch_force_final(c->imp, c->partial, nb, c->dxyz, c->mAg, c->st));
@@ -879,7 +879,7 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
     const bool mx = c->mixed;
     switch (which) {
     case HETD_F_PSI: src = c->psi; bytes = nC; break;
-    case HETD_F_DEN: src = mx ? widen(c, c->den32, c->n, c->tmp) : c->den; break;
+    case HETD_F_DEN: src = c->den; break;
     case HETD_F_POT4: src = mx ? widen(c, c->pot4_32, c->n, c->tmp) : c->pot4; break;
     case HETD_F_DENCG: src = mx ? widen(c, c->dencg32, c->n, c->tmp) : c->dencg; break;
     case HETD_F_DELJ4: src = mx ? widen(c, c->delj4_32, c->n, c->tmp) : c->delj4; break;
@@ -906,7 +906,6 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
     case HETD_F_DXDEN:
     case HETD_F_DYDEN:
     case HETD_F_DZDEN:
-        if (mx) HCHK(c, launch_to_double(c->den32, c->den, c->n, c->st));
         if ((rc = deriv_dev(c, 1, which - HETD_F_DXDEN + 1, 0, c->den, c->tmp))) return rc;
         src = c->tmp;
         break;
@@ -917,8 +916,7 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
     case HETD_F_PENALTY: {
         if (!c->p.lsolid) return fail(c, HETD_ERR_ARG, "no penalty without lsolid");
         std::vector<double> d(c->n);
-        if (mx) HCHK(c, launch_to_double(c->den32, c->tmp, c->n, c->st));
-        HCHK(c, hipMemcpyAsync(d.data(), mx ? c->tmp : c->den, nR, hipMemcpyDeviceToHost, c->st));
+        HCHK(c, hipMemcpyAsync(d.data(), c->den, nR, hipMemcpyDeviceToHost, c->st));
         HCHK(c, hipStreamSynchronize(c->st));
         for (long i = 0; i < c->n; ++i) {
             const double t = std::tanh(c->p.beta_solid * (d[i] - c->p.den_m));

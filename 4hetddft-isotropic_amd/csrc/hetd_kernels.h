@@ -104,7 +104,7 @@ struct UpdateArgs {
     double* partial;  // CORR: per-block sum |pc|
     int* guard;       // RK: set if den not > 0
     const float* pot4f;  // mixed precision: FP32 mean field (else pot4)
-    float* denf;         // mixed precision: FP32 density output (else den)
+    float* denf;         // mixed precision: FP32 copy of den for the mean field
 };
 
 hipError_t launch_update(const UpdateArgs& a, hipStream_t st, int* nblocks_out);
@@ -127,8 +127,7 @@ struct ForceArgs {
     StencilCoef sc;
     const double* den;
     const ImpState* imp;
-    double* partial;    // 3 per block
-    const float* denf;  // mixed precision: FP32 density, FP32 stencil
+    double* partial;  // 3 per block
 };
 hipError_t launch_force_partial(const ForceArgs& a, hipStream_t st, int* nblocks);
 

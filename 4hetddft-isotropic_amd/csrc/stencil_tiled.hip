@@ -1,10 +1,10 @@
 // stencil_tiled.hip -- LDS-tiled 2.5-D stencil engine for gfx950.
 //
-// A workgroup owns a 64 (x) x 8 (y) column of the grid and marches it through
-// a chunk of z planes.  Per plane the 64x8 tile plus an M-point halo in x and
-// y sits in LDS (x and y stencils read LDS); each thread keeps the z column of
-// its two points in a register queue of 2M+1 values (the z stencil reads
-// registers), so every grid value is fetched from HBM about once.  Tiles are
+// A workgroup owns a 64 (x) x TTY (y) column of the grid and marches it
+// through a chunk of z planes.  Per plane the tile plus an M-point halo in x
+// and y sits in LDS (x and y stencils read LDS); each thread keeps the z
+// column of its TPY points in a register queue of 2M+1 values (the z stencil
+// reads registers), so every grid value is fetched from HBM about once.  Tiles are
 // ordered so that consecutive tiles, which share halos, run on the same XCD
 // (blocks are dealt to the 8 XCDs round-robin).
 //
@@ -19,7 +19,7 @@
 
 namespace hetd {
 
-constexpr int TTX = 64, TTY = 8, TNT = 256, TPY = 2;
+constexpr int TTX = 64;
 
 template <typename V> struct vz;
 template <> struct vz<float> {
@@ -66,13 +66,12 @@ __device__ __forceinline__ V load_bc(const V* __restrict__ f, int nx, int ny, in
     return sg == 1.0 ? v : vz<V>::scale(v, sg);
 }
 
-enum TKind { TK_UPDATE = 0, TK_FORCE = 1, TK_ENERGY = 2, TK_FORCE32 = 3 };
+enum TKind { TK_UPDATE = 0, TK_FORCE = 1, TK_ENERGY = 2 };
 
 template <int KIND> struct TArgs;
 template <> struct TArgs<TK_UPDATE> { using V = double2; using A = UpdateArgs; static constexpr int NC = 1; };
 template <> struct TArgs<TK_FORCE> { using V = double; using A = ForceArgs; static constexpr int NC = 3; };
 template <> struct TArgs<TK_ENERGY> { using V = double2; using A = EnergyArgs; static constexpr int NC = EN_NCOMP; };
-template <> struct TArgs<TK_FORCE32> { using V = float; using A = ForceArgs; static constexpr int NC = 3; };
 
 struct TiledCommon {
     Grid g;
@@ -96,10 +95,14 @@ __device__ __forceinline__ V cell_value(const V* __restrict__ f, int nx, int ny,
     return load_bc<V>(f, nx, ny, nz, gx, gy, iz, icon);
 }
 
-template <int KIND, int M>
+// TPY points per thread along y, TNT threads: tile 64 x TTY with
+// TTY = TPY*TNT/64; PF: software-pipelined halo / queue loads
+template <int KIND, int M, int TPY, int TNT, bool PF>
 __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KIND>::A args) {
     using V = typename TArgs<KIND>::V;
     constexpr int NC = TArgs<KIND>::NC;
+    constexpr int RS = TNT / TTX;        // thread rows
+    constexpr int TTY = TPY * RS;
     constexpr int LX = TTX + 2 * M, LY = TTY + 2 * M, W = 2 * M + 1;
     __shared__ V lds[LX * LY];
     __shared__ double red[NC * TNT / 64];
@@ -116,7 +119,7 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
     const int tile_x = t % tc.ntx, tile_y = (t / tc.ntx) % tc.nty, tile_z = t / (tc.ntx * tc.nty);
     const int ix0 = tile_x * TTX, iy0 = tile_y * TTY, iz0 = tile_z * tc.zc;
     const int izend = min(iz0 + tc.zc, nz);
-    const int tx = threadIdx.x % TTX, ty = threadIdx.x / TTX;  // ty 0..3
+    const int tx = threadIdx.x % TTX, ty = threadIdx.x / TTX;
     const int ix = ix0 + tx;
     const V* __restrict__ f;
     const double* c1;
@@ -127,9 +130,6 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
         c1 = tc.sc.c2[0]; c2 = tc.sc.c2[1]; c3 = tc.sc.c2[2];
     } else if constexpr (KIND == TK_FORCE) {
         f = args.den;
-        c1 = tc.sc.c1[0]; c2 = tc.sc.c1[1]; c3 = tc.sc.c1[2];
-    } else if constexpr (KIND == TK_FORCE32) {
-        f = args.denf;
         c1 = tc.sc.c1[0]; c2 = tc.sc.c1[1]; c3 = tc.sc.c1[2];
     } else {
         f = args.psi;
@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
     V q[TPY][W];
 #pragma unroll
     for (int p = 0; p < TPY; ++p) {
-        iy[p] = iy0 + ty + 4 * p;
+        iy[p] = iy0 + ty + RS * p;
         valid[p] = ix < nx && iy[p] < ny;
 #pragma unroll
         for (int k = 0; k < W - 1; ++k) {
@@ -155,40 +155,82 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
         }
     }
 
-    for (int iz = iz0; iz < izend; ++iz) {
+    // halo cells of this thread (slot k: cell threadIdx.x + k*TNT of the
+    // ring); x halo = M columns on each side of the tile rows, y halo = M
+    // rows above and below the tile columns (corners are never read)
+    constexpr int NHX = 2 * M * TTY, NH = NHX + 2 * M * TTX, KH = (NH + TNT - 1) / TNT;
+    int hl[KH];  // LDS index of slot k, -1 if unused
+    int hgx[KH], hgy[KH];
 #pragma unroll
-        for (int p = 0; p < TPY; ++p) {
-            const int zz = iz + M;
-            q[p][W - 1] = (valid[p] && (zok || zz < nz)) ? load_bc<V>(f, nx, ny, nz, ix, iy[p], zz, icon)
-                                                          : vz<V>::zero();
-        }
-        __syncthreads();
-        // plane iz into LDS: interior from the queues, halo from memory
-#pragma unroll
-        for (int p = 0; p < TPY; ++p)
-            lds[(ty + 4 * p + M) * LX + tx + M] =
-                valid[p] ? q[p][M] : cell_value<V, M>(f, nx, ny, nz, ix, iy[p], iz, icon);
-        constexpr int NHX = 2 * M * TTY, NH = NHX + 2 * M * TTX;
-        for (int h = threadIdx.x; h < NH; h += TNT) {
+    for (int k = 0; k < KH; ++k) {
+        const int h = threadIdx.x + k * TNT;
+        hl[k] = -1;
+        if (h < NH) {
             int lx, ly;
-            if (h < NHX) {  // x halo: M columns on each side of the tile rows
+            if (h < NHX) {
                 const int r = h / (2 * M), cc = h % (2 * M);
                 ly = r + M;
                 lx = cc < M ? cc : TTX + cc;
-            } else {        // y halo: M rows above and below the tile columns
+            } else {
                 const int hh = h - NHX;
                 const int r = hh / TTX, cc = hh % TTX;
                 lx = cc + M;
                 ly = r < M ? r : TTY + r;
             }
-            const int gx = ix0 + lx - M, gy = iy0 + ly - M;
-            lds[ly * LX + lx] = cell_value<V, M>(f, nx, ny, nz, gx, gy, iz, icon);
+            hl[k] = ly * LX + lx;
+            hgx[k] = ix0 + lx - M;
+            hgy[k] = iy0 + ly - M;
+        }
+    }
+    // software pipeline: the halo of plane iz and the queue fronts are
+    // loaded one iteration ahead, while plane iz-1 is being computed
+    V hreg[KH], qn[TPY];
+#pragma unroll
+    for (int k = 0; k < KH; ++k)
+        hreg[k] = PF && hl[k] >= 0 && iz0 < izend ? cell_value<V, M>(f, nx, ny, nz, hgx[k], hgy[k], iz0, icon)
+                                                  : vz<V>::zero();
+#pragma unroll
+    for (int p = 0; p < TPY; ++p) {
+        const int zz = iz0 + M;
+        qn[p] = (PF && valid[p] && (zok || zz < nz)) ? load_bc<V>(f, nx, ny, nz, ix, iy[p], zz, icon)
+                                                     : vz<V>::zero();
+    }
+
+    for (int iz = iz0; iz < izend; ++iz) {
+#pragma unroll
+        for (int p = 0; p < TPY; ++p) {
+            if (!PF) {
+                const int zz = iz + M;
+                qn[p] = (valid[p] && (zok || zz < nz)) ? load_bc<V>(f, nx, ny, nz, ix, iy[p], zz, icon)
+                                                        : vz<V>::zero();
+            }
+            q[p][W - 1] = qn[p];
         }
         __syncthreads();
+        // plane iz into LDS: interior from the queues, halo from the prefetch
+#pragma unroll
+        for (int p = 0; p < TPY; ++p)
+            lds[(ty + RS * p + M) * LX + tx + M] =
+                valid[p] ? q[p][M] : cell_value<V, M>(f, nx, ny, nz, ix, iy[p], iz, icon);
+#pragma unroll
+        for (int k = 0; k < KH; ++k)
+            if (hl[k] >= 0) lds[hl[k]] = PF ? hreg[k] : cell_value<V, M>(f, nx, ny, nz, hgx[k], hgy[k], iz, icon);
+        __syncthreads();
+        if (PF && iz + 1 < izend) {
+#pragma unroll
+            for (int k = 0; k < KH; ++k)
+                if (hl[k] >= 0) hreg[k] = cell_value<V, M>(f, nx, ny, nz, hgx[k], hgy[k], iz + 1, icon);
+#pragma unroll
+            for (int p = 0; p < TPY; ++p) {
+                const int zz = iz + 1 + M;
+                qn[p] = (valid[p] && (zok || zz < nz)) ? load_bc<V>(f, nx, ny, nz, ix, iy[p], zz, icon)
+                                                        : vz<V>::zero();
+            }
+        }
 #pragma unroll
         for (int p = 0; p < TPY; ++p) {
             if (!valid[p]) continue;
-            const int ly = ty + 4 * p + M;
+            const int ly = ty + RS * p + M;
             V sx = vz<V>::zero(), sy = vz<V>::zero(), sz = vz<V>::zero();
             if (nx >= 2 * M + 1)
 #pragma unroll
@@ -229,7 +271,7 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     const double h = hypot(pn.x, pn.y), d = h * h;
                     a.psi_out[o] = pn;
                     if (a.denf) a.denf[o] = (float)d;
-                    else a.den[o] = d;
+                    a.den[o] = d;
                     if (!(d > 0.0)) atomicOr(a.guard, 1);
                 } else if (a.kind == STAGE_PRED) {
                     const double c4 = (4.0 / 3.0) * a.dt;
@@ -243,9 +285,9 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     a.psi_out[o] = pn;
                     a.pc[o] = P;
                     a.hnew[o] = fv;
-                    const double h = hypot(pn.x, pn.y);
-                    if (a.denf) a.denf[o] = (float)(h * h);
-                    else a.den[o] = h * h;
+                    const double h = hypot(pn.x, pn.y), d = h * h;
+                    if (a.denf) a.denf[o] = (float)d;
+                    a.den[o] = d;
                 } else {
                     const double2 p3 = a.po3[o], p2 = a.po2[o], h1 = a.h1[o], h2 = a.h2[o], pc = a.pc[o];
                     const double t3 = 3.0 * a.dt;
@@ -259,12 +301,12 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     pn.y = Cc.y + c9 * pcn.y;
                     a.pc[o] = pcn;
                     a.psi_out[o] = pn;
-                    const double h = hypot(pn.x, pn.y);
-                    if (a.denf) a.denf[o] = (float)(h * h);
-                    else a.den[o] = h * h;
+                    const double h = hypot(pn.x, pn.y), d = h * h;
+                    if (a.denf) a.denf[o] = (float)d;
+                    a.den[o] = d;
                     acc[0] += hypot(pcn.x, pcn.y);
                 }
-            } else if constexpr (KIND == TK_FORCE || KIND == TK_FORCE32) {
+            } else if constexpr (KIND == TK_FORCE) {
                 acc[0] += (double)sx * u;
                 acc[1] += (double)sy * u;
                 acc[2] += (double)sz * u;
@@ -304,12 +346,12 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
 
 // ------------------------------------------------------------- launchers
 
-static TiledCommon tiled_common(const Grid& g, const StencilCoef& sc) {
+static TiledCommon tiled_common(const Grid& g, const StencilCoef& sc, int tty) {
     TiledCommon tc;
     tc.g = g;
     tc.sc = sc;
     tc.ntx = (g.nx + TTX - 1) / TTX;
-    tc.nty = (g.ny + TTY - 1) / TTY;
+    tc.nty = (g.ny + tty - 1) / tty;
     const long nbxy = (long)tc.ntx * tc.nty;
     // aim at >= ~1024 workgroups (256 CUs), z chunks of 4..64 planes
     long zc = (g.nz * nbxy + 1023) / 1024;
@@ -322,22 +364,50 @@ static TiledCommon tiled_common(const Grid& g, const StencilCoef& sc) {
     return tc;
 }
 
-int tiled_blocks(const Grid& g, const StencilCoef& sc) {
-    TiledCommon tc = tiled_common(g, sc);
-    return tc.ntx * tc.nty * tc.ntz;
+bool tiled_supported(const StencilCoef& sc) { return sc.m == 4 || sc.m == 6; }
+
+// tile variants (points per thread, threads, pipelined loads).  Variant 2
+// (one point per thread, 512 threads, 64x8 tile) was fastest for all three
+// kinds on MI355X at 256^3: its register queue leaves room for 3-4 waves per
+// SIMD, where two points per thread cap the complex kinds at 1-2.
+// HETD_TILE_<KIND>=0..3 selects another variant.
+struct TileVariant { int tpy, tnt; bool pf; };
+static constexpr TileVariant kVariants[4] = {{2, 256, false}, {2, 256, true}, {1, 512, false}, {4, 256, true}};
+
+static int variant_for(int kind) {
+    static const char* names[3] = {"HETD_TILE_UPDATE", "HETD_TILE_FORCE", "HETD_TILE_ENERGY"};
+    const char* e = getenv(names[kind]);
+    if (e && e[0] >= '0' && e[0] <= '3') return e[0] - '0';
+    return 2;
 }
 
-bool tiled_supported(const StencilCoef& sc) { return sc.m == 4 || sc.m == 6; }
+template <int KIND, int M, int V>
+static void launch_v(const TiledCommon& tc, const typename TArgs<KIND>::A& a, int nlaunch, hipStream_t st) {
+    constexpr TileVariant tv = kVariants[V];
+    hipLaunchKernelGGL((k_tiled<KIND, M, tv.tpy, tv.tnt, tv.pf>), dim3(nlaunch), dim3(tv.tnt), 0, st, tc, a);
+}
+
+template <int KIND, int M>
+static void launch_m(int v, const TiledCommon& tc, const typename TArgs<KIND>::A& a, int nlaunch, hipStream_t st) {
+    switch (v) {
+    case 0: launch_v<KIND, M, 0>(tc, a, nlaunch, st); break;
+    case 1: launch_v<KIND, M, 1>(tc, a, nlaunch, st); break;
+    case 2: launch_v<KIND, M, 2>(tc, a, nlaunch, st); break;
+    default: launch_v<KIND, M, 3>(tc, a, nlaunch, st); break;
+    }
+}
 
 template <int KIND>
 static hipError_t launch_tiled(const typename TArgs<KIND>::A& a, hipStream_t st, int* nblocks) {
-    TiledCommon tc = tiled_common(a.g, a.sc);
+    const int v = variant_for(KIND);
+    const TileVariant tv = kVariants[v];
+    TiledCommon tc = tiled_common(a.g, a.sc, tv.tpy * tv.tnt / TTX);
     const int nb = tc.ntx * tc.nty * tc.ntz;
     const int nlaunch = 8 * ((nb + 7) / 8);
     if (nblocks) *nblocks = nb;
     switch (a.sc.m) {
-    case 4: hipLaunchKernelGGL((k_tiled<KIND, 4>), dim3(nlaunch), dim3(TNT), 0, st, tc, a); break;
-    case 6: hipLaunchKernelGGL((k_tiled<KIND, 6>), dim3(nlaunch), dim3(TNT), 0, st, tc, a); break;
+    case 4: launch_m<KIND, 4>(v, tc, a, nlaunch, st); break;
+    case 6: launch_m<KIND, 6>(v, tc, a, nlaunch, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -347,7 +417,7 @@ hipError_t launch_update_tiled(const UpdateArgs& a, hipStream_t st, int* nblocks
     return launch_tiled<TK_UPDATE>(a, st, nblocks);
 }
 hipError_t launch_force_tiled(const ForceArgs& a, hipStream_t st, int* nblocks) {
-    return a.denf ? launch_tiled<TK_FORCE32>(a, st, nblocks) : launch_tiled<TK_FORCE>(a, st, nblocks);
+    return launch_tiled<TK_FORCE>(a, st, nblocks);
 }
 hipError_t launch_energy_tiled(const EnergyArgs& a, hipStream_t st, int* nblocks) {
     return launch_tiled<TK_ENERGY>(a, st, nblocks);
