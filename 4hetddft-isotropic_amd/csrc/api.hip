@@ -41,6 +41,9 @@ struct hetd_ctx {
     double *s1 = nullptr, *s2 = nullptr, *s3 = nullptr, *s4 = nullptr, *s5 = nullptr, *s6 = nullptr;
     // mixed precision (HETD_MIXED): FP32 density, mean field and FFT work
     bool mixed = false;
+    // delj4 of the last poten: DELJ4_DEN = not formed (make it from den),
+    // DELJ4_OK = formed, DELJ4_SPEC = its z spectrum waits in B
+    int delj4_state = 0;
     float *den32 = nullptr, *pot4_32 = nullptr, *dencg32 = nullptr, *delj4_32 = nullptr, *tmp32 = nullptr;
     float2 *A32 = nullptr, *B32 = nullptr, *C32 = nullptr;
     float *fvlj32 = nullptr, *wcgk32 = nullptr, *kalfs32 = nullptr;
@@ -150,7 +153,8 @@ template <typename T> static hipError_t run_y(hetd_ctx* c, YPassArgs<T> y, int n
 template <typename T> static hipError_t run_z(hetd_ctx* c, ZPassArgs<T> z, int nbuf) {
     const double s = (double)sizeof(T);
     double per = 4 * s;
-    if (z.mode == ZMODE_CONV) per = 2 * s + 3 * s * z.nk + (z.fwd_out ? 2 * s : 0.0);
+    if (z.mode == ZMODE_CONV)
+        per = 2 * s + 3 * s * z.nk + (z.fwd_out ? 2 * s : 0.0) + (z.kout ? 3 * s : 0.0) + (z.addin ? 2 * s : 0.0);
     tbeg(c, HETD_K_ZPASS);
     hipError_t e = reg_fft_supported(z.ax.n) ? launch_zreg(z, nbuf, c->st) : launch_zpass(z, nbuf, c->st);
     tend(c, HETD_K_ZPASS, per * c->nh * nbuf);
@@ -274,6 +278,8 @@ static int term_alfa_dev(hetd_ctx* c) {
     return 0;
 }
 
+enum { DELJ4_DEN = 0, DELJ4_OK = 1, DELJ4_SPEC = 2 };
+
 // the density / mean-field / FFT buffers of one precision
 template <typename T> struct PF {
     T *den, *pot4, *dencg, *delj4, *tmp;
@@ -288,12 +294,19 @@ static PF<float> pf32(hetd_ctx* c) {
                      c->fvlj32, c->wcgk32, c->kalfs32};
 }
 
-// poten() (poten.f90:5-118) up to pot4 without the alpha_s term: den ->
-// pot4, dencg, delj4 (and, for OTC, the Gaussian spectrum in Cs).
-// want_ccor / want_fden also store the correlation term in tmp and the
-// forward spectrum of den in Cs.
+// poten() (poten.f90:5-118) up to pot4 without the alpha_s term.
+//
+// Stepping form (full = false): the two linear convolutions that enter pot4
+// additively, delj4 = IFFT(V_LJ fden) and sto1 = IFFT(W FFT(s)), are summed
+// in the spectrum and inverted once: den -> A = W fden (inverted to dencg),
+// B = V_LJ fden (kept as a z spectrum); s -> A = W FFT(s) + B -> pot4.
+// delj4 itself is then not formed; ensure_delj4 makes it when asked for.
+//
+// Full form (full = true): the reference's sequence of transforms, storing
+// delj4; want_ccor stores sto1 in tmp, want_fden the spectrum of den in Cs,
+// keep_pot4 = false leaves pot4 untouched (fields recomputed for a copy).
 template <typename T>
-static int poten_t(hetd_ctx* c, const PF<T>& f, bool want_ccor, bool want_fden) {
+static int poten_t(hetd_ctx* c, const PF<T>& f, bool full, bool want_ccor, bool want_fden, bool keep_pot4) {
     const long nl = (long)c->ny * c->nz;
     XPassArgs<T> x{};
     x.ax = c->ax[0]; x.nlines = nl; x.mode = XMODE_R2C; x.rin = f.den; x.cout = f.A;
@@ -304,18 +317,28 @@ static int poten_t(hetd_ctx* c, const PF<T>& f, bool want_ccor, bool want_fden) 
     // fden*W -> A, fden*V_LJ -> B [, fden*K_alpha -> Cs] (poten.f90:34-50, 96-101)
     ZPassArgs<T> z{};
     z.ax = c->ax[2]; z.nxh = c->nxh; z.ny = c->ny; z.mode = ZMODE_CONV; z.in[0] = f.A;
-    z.nk = c->otc ? 3 : 2;
-    z.out[0][0] = f.A; z.out[0][1] = f.B; z.out[0][2] = f.Cs;
-    z.kern[0] = f.wcgk; z.kern[1] = f.fvlj; z.kern[2] = f.kalfs;
-    if (want_fden && c->otc) return fail(c, HETD_ERR_STATE, "fden copy not available with OTC");
-    z.fwd_out = want_fden ? f.Cs : nullptr;
+    z.kern[0] = f.wcgk; z.out[0][0] = f.A;
+    if (full) {
+        z.nk = c->otc ? 3 : 2;
+        z.out[0][1] = f.B; z.out[0][2] = f.Cs;
+        z.kern[1] = f.fvlj; z.kern[2] = f.kalfs;
+        if (want_fden && c->otc) return fail(c, HETD_ERR_STATE, "fden copy not available with OTC");
+        z.fwd_out = want_fden ? f.Cs : nullptr;
+    } else {
+        z.nk = 1;
+        z.kout = f.B; z.kkern = f.fvlj;
+    }
     HCHK(c, run_z(c, z, 1));
     y.inv = 1; y.buf[0] = f.A; y.buf[1] = f.B; y.buf[2] = f.Cs;
-    HCHK(c, run_y(c, y, c->otc ? 3 : 2));
-    // delj4 = c2r(fden*V)*renor
-    XPassArgs<T> xl{};
-    xl.ax = c->ax[0]; xl.nlines = nl; xl.mode = XMODE_C2R; xl.cin = f.B; xl.rout = f.delj4; xl.scale = (T)c->renor;
-    HCHK(c, run_x(c, xl));
+    HCHK(c, run_y(c, y, full ? (c->otc ? 3 : 2) : 1));
+    if (full) {
+        // delj4 = c2r(fden*V)*renor
+        XPassArgs<T> xl{};
+        xl.ax = c->ax[0]; xl.nlines = nl; xl.mode = XMODE_C2R; xl.cin = f.B; xl.rout = f.delj4;
+        xl.scale = (T)c->renor;
+        HCHK(c, run_x(c, xl));
+    }
+    c->delj4_state = full ? DELJ4_OK : DELJ4_SPEC;
     // dencg = c2r(fden*W)*renor, s = den*dencg*(cp4+cpp4*dencg), A = x-r2c(s)
     XPassArgs<T> xc{};
     xc.ax = c->ax[0]; xc.nlines = nl; xc.mode = XMODE_CG_S; xc.cin = f.A; xc.rin = f.den; xc.rout = f.dencg;
@@ -326,13 +349,15 @@ static int poten_t(hetd_ctx* c, const PF<T>& f, bool want_ccor, bool want_fden) 
     ZPassArgs<T> z2{};
     z2.ax = c->ax[2]; z2.nxh = c->nxh; z2.ny = c->ny; z2.mode = ZMODE_CONV; z2.nk = 1; z2.in[0] = f.A;
     z2.out[0][0] = f.A; z2.kern[0] = f.wcgk;
+    if (!full) z2.addin = f.B;
     HCHK(c, run_z(c, z2, 1));
     y.inv = 1;
     HCHK(c, run_y(c, y, 1));
     // pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + sto1 [+ penalty] (poten.f90:80-94, 110-114)
     XPassArgs<T> xp{};
     xp.ax = c->ax[0]; xp.nlines = nl; xp.mode = XMODE_POT; xp.cin = f.A; xp.rin = f.den; xp.rin2 = f.dencg;
-    xp.rin3 = f.delj4; xp.rout = f.pot4; xp.rout2 = want_ccor ? f.tmp : nullptr; xp.scale = (T)c->renor;
+    xp.rin3 = full ? f.delj4 : nullptr; xp.rout = keep_pot4 ? f.pot4 : nullptr;
+    xp.rout2 = want_ccor ? f.tmp : nullptr; xp.scale = (T)c->renor;
     xp.a0 = (T)(c->p.cp4 / 2.0); xp.a1 = (T)(c->p.cpp4 / 3.0);
     xp.lsolid = c->p.lsolid && !c->otc;
     xp.csol = (T)c->p.c_solid; xp.beta = (T)c->p.beta_solid; xp.den_m = (T)c->p.den_m;
@@ -340,11 +365,50 @@ static int poten_t(hetd_ctx* c, const PF<T>& f, bool want_ccor, bool want_fden) 
     return 0;
 }
 
-static int poten_dev(hetd_ctx* c, bool want_ccor, bool want_fden) {
-    if (c->mixed) return poten_t<float>(c, pf32(c), want_ccor, want_fden);
-    int rc = poten_t<double>(c, pf64(c), want_ccor, want_fden);
+// delj4 = c2r(V_LJ r2c(den)) * renor (poten.f90:38-45) for the energy and
+// field copies after stepping-form potentials: from the spectrum V_LJ fden
+// the last poten left in B, or (no poten since the state was set) from den
+template <typename T> static int delj4_t(hetd_ctx* c, const PF<T>& f, bool from_b) {
+    const long nl = (long)c->ny * c->nz;
+    YPassArgs<T> y{};
+    y.ax = c->ax[1]; y.nxh = c->nxh; y.buf[0] = f.B;
+    ZPassArgs<T> z{};
+    z.ax = c->ax[2]; z.nxh = c->nxh; z.ny = c->ny; z.in[0] = f.B; z.out[0][0] = f.B;
+    if (from_b) {
+        z.mode = ZMODE_INV;
+    } else {
+        XPassArgs<T> x{};
+        x.ax = c->ax[0]; x.nlines = nl; x.mode = XMODE_R2C; x.rin = f.den; x.cout = f.B;
+        HCHK(c, run_x(c, x));
+        y.inv = 0;
+        HCHK(c, run_y(c, y, 1));
+        z.mode = ZMODE_CONV; z.nk = 1; z.kern[0] = f.fvlj;
+    }
+    HCHK(c, run_z(c, z, 1));
+    y.inv = 1;
+    HCHK(c, run_y(c, y, 1));
+    XPassArgs<T> xl{};
+    xl.ax = c->ax[0]; xl.nlines = nl; xl.mode = XMODE_C2R; xl.cin = f.B; xl.rout = f.delj4; xl.scale = (T)c->renor;
+    HCHK(c, run_x(c, xl));
+    return 0;
+}
+
+static int ensure_delj4(hetd_ctx* c) {
+    if (c->delj4_state == DELJ4_OK) return 0;
+    const bool from_b = c->delj4_state == DELJ4_SPEC;
+    const int rc = c->mixed ? delj4_t<float>(c, pf32(c), from_b) : delj4_t<double>(c, pf64(c), from_b);
     if (rc) return rc;
-    if (c->otc) {
+    c->delj4_state = DELJ4_OK;
+    return 0;
+}
+
+static int poten_dev(hetd_ctx* c, bool want_ccor = false, bool want_fden = false, bool keep_pot4 = true) {
+    // OTC keeps the full form: its K_alpha spectrum rides on the first z pass
+    const bool full = want_ccor || want_fden || c->otc;
+    if (c->mixed) return poten_t<float>(c, pf32(c), full, want_ccor, want_fden, keep_pot4);
+    int rc = poten_t<double>(c, pf64(c), full, want_ccor, want_fden, keep_pot4);
+    if (rc) return rc;
+    if (c->otc && keep_pot4) {
         XPassArgs<double> xa{};
         xa.ax = c->ax[0]; xa.nlines = (long)c->ny * c->nz; xa.mode = XMODE_C2R; xa.cin = c->Cs; xa.rout = c->denalf;
         xa.scale = c->renor;
@@ -697,6 +761,7 @@ int hetd_set_state(hetd_ctx* c, const double* psi, const double r[3], const doub
     HCHK(c, hipMemcpyAsync(c->psi, psi, nC, hipMemcpyHostToDevice, c->st));
     HCHK(c, launch_den_from_psi(c->psi, c->den, c->n, c->st));
     if (c->mixed) HCHK(c, launch_den32_from_psi(c->psi, c->den32, c->n, c->st));
+    c->delj4_state = DELJ4_DEN;
     HCHK(c, hipMemsetAsync(c->q, 0, nC, c->st));
     HCHK(c, hipMemsetAsync(c->pc, 0, nC, c->st));
     for (int i = 0; i < 3; ++i) HCHK(c, hipMemsetAsync(c->psiold[i], 0, nC, c->st));
@@ -808,6 +873,7 @@ int hetd_evolve(hetd_ctx* c, double dt, int32_t nsteps, double* traj) {
 int hetd_energy(hetd_ctx* c, hetd_energies* e) {
     if (!c || !e) return HETD_ERR_ARG;
     if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
+    if (int rc = ensure_delj4(c)) return rc;
     EnergyArgs a{};
     a.g = c->grid; a.sc = c->sc; a.psi = c->psi; a.den = c->den; a.dencg = c->dencg; a.delj4 = c->delj4;
     a.imp = c->imp; a.aux1 = 0.5 * c->p.cp4; a.aux2 = (1.0 / 3.0) * c->p.cpp4; a.beta = c->p.beta_solid;
@@ -882,18 +948,23 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
     case HETD_F_DEN: src = c->den; break;
     case HETD_F_POT4: src = mx ? widen(c, c->pot4_32, c->n, c->tmp) : c->pot4; break;
     case HETD_F_DENCG: src = mx ? widen(c, c->dencg32, c->n, c->tmp) : c->dencg; break;
-    case HETD_F_DELJ4: src = mx ? widen(c, c->delj4_32, c->n, c->tmp) : c->delj4; break;
+    case HETD_F_DELJ4:
+        if ((rc = ensure_delj4(c))) return rc;
+        src = mx ? widen(c, c->delj4_32, c->n, c->tmp) : c->delj4;
+        break;
     case HETD_F_PC: src = c->pc; bytes = nC; break;
     case HETD_F_Q: src = c->q; bytes = nC; break;
     case HETD_F_FVLJ: src = c->fvlj; bytes = nHr; break;
     case HETD_F_WCGK: src = c->wcgk; bytes = nHr; break;
     case HETD_F_CCOR:
     case HETD_F_FDEN: {
-        // recompute the last poten (den is unchanged since) keeping the
-        // intermediate; pot4/dencg/delj4 are reproduced bit for bit
+        // recompute the last poten in its full form (den is unchanged since)
+        // keeping the intermediate; dencg and delj4 come out bit for bit as
+        // the stepping form makes them, pot4 is left as it is (OTC, whose
+        // stepping form is the full one, rewrites it bit for bit)
         if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
         if (which == HETD_F_FDEN && c->otc) return fail(c, HETD_ERR_ARG, "FDEN not kept with OTC");
-        if ((rc = poten_dev(c, which == HETD_F_CCOR, which == HETD_F_FDEN))) return rc;
+        if ((rc = poten_dev(c, which == HETD_F_CCOR, which == HETD_F_FDEN, c->otc))) return rc;
         if (which == HETD_F_CCOR) {
             src = mx ? widen(c, c->tmp32, c->n, c->tmp) : (const void*)c->tmp;
         } else {

@@ -85,24 +85,24 @@ __global__ void __launch_bounds__(256) k_xpass(XPassArgs<T> a) {
                 T ca = va, cb = vb;
                 {
                     const T g = a.rin2[ia];
-                    T u = a.rin3[ia] + g * g * (a.a0 + a.a1 * g) + ca;
+                    T u = (a.rin3 ? a.rin3[ia] : (T)0) + g * g * (a.a0 + a.a1 * g) + ca;
                     if (a.lsolid) {
                         const T d = a.rin[ia];
                         const T t = tanh(a.beta * (d - a.den_m));
                         u = u + a.csol * ((T)1 + t + a.beta * d * ((T)1 - t * t));
                     }
-                    a.rout[ia] = u;
+                    if (a.rout) a.rout[ia] = u;
                     if (a.rout2) a.rout2[ia] = ca;
                 }
                 if (hb) {
                     const T g = a.rin2[ib];
-                    T u = a.rin3[ib] + g * g * (a.a0 + a.a1 * g) + cb;
+                    T u = (a.rin3 ? a.rin3[ib] : (T)0) + g * g * (a.a0 + a.a1 * g) + cb;
                     if (a.lsolid) {
                         const T d = a.rin[ib];
                         const T t = tanh(a.beta * (d - a.den_m));
                         u = u + a.csol * ((T)1 + t + a.beta * d * ((T)1 - t * t));
                     }
-                    a.rout[ib] = u;
+                    if (a.rout) a.rout[ib] = u;
                     if (a.rout2) a.rout2[ib] = cb;
                 }
             }
@@ -187,12 +187,20 @@ __global__ void __launch_bounds__(256) k_zpass(ZPassArgs<T> a) {
         }
         if (a.mode == ZMODE_FWD) return;
     }
+    if (a.kout)
+        for (int idx = tid; idx < nz * TX; idx += nthr) {
+            const int j = idx / TX, t = idx % TX;
+            if (kx0 + t < nxh) a.kout[base + zs * j + t] = cscale(l0[idx], a.kkern[base + zs * j + t]);
+        }
     for (int q = 0; q < a.nk; ++q) {
         const T* K = a.kern[q];
         for (int idx = tid; idx < nz * TX; idx += nthr) {
             const int j = idx / TX, t = idx % TX;
-            const T kv = (kx0 + t) < nxh ? K[base + zs * j + t] : (T)0;
-            l1[idx] = cscale(l0[idx], kv);
+            const bool in_ = (kx0 + t) < nxh;
+            const T kv = in_ ? K[base + zs * j + t] : (T)0;
+            C w = cscale(l0[idx], kv);
+            if (q == 0 && a.addin && in_) w = cadd(w, a.addin[base + zs * j + t]);
+            l1[idx] = w;
         }
         __syncthreads();
         fft_lds<T, PB, true>(l1, a.ax, TX, TX, 1, true, tid, nthr);

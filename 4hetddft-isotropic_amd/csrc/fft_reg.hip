@@ -178,22 +178,22 @@ __global__ void __launch_bounds__(R* TL) k_xreg(XPassArgs<T> a) {
             } else {  // POT
                 if (va) {
                     const T g = a.rin2[ia];
-                    T u = a.rin3[ia] + g * g * (a.a0 + a.a1 * g) + ra;
+                    T u = (a.rin3 ? a.rin3[ia] : (T)0) + g * g * (a.a0 + a.a1 * g) + ra;
                     if (a.lsolid) {
                         const T d = a.rin[ia], t = tanh(a.beta * (d - a.den_m));
                         u = u + a.csol * ((T)1 + t + a.beta * d * ((T)1 - t * t));
                     }
-                    a.rout[ia] = u;
+                    if (a.rout) a.rout[ia] = u;
                     if (a.rout2) a.rout2[ia] = ra;
                 }
                 if (vb) {
                     const T g = a.rin2[ib];
-                    T u = a.rin3[ib] + g * g * (a.a0 + a.a1 * g) + rb;
+                    T u = (a.rin3 ? a.rin3[ib] : (T)0) + g * g * (a.a0 + a.a1 * g) + rb;
                     if (a.lsolid) {
                         const T d = a.rin[ib], t = tanh(a.beta * (d - a.den_m));
                         u = u + a.csol * ((T)1 + t + a.beta * d * ((T)1 - t * t));
                     }
-                    a.rout[ib] = u;
+                    if (a.rout) a.rout[ib] = u;
                     if (a.rout2) a.rout2[ib] = rb;
                 }
             }
@@ -256,7 +256,9 @@ __global__ void __launch_bounds__(R* TX) k_yreg(YPassArgs<T> a) {
 
 // ------------------------------------------------------------------ z pass
 
-template <typename T, int R, int TX>
+// NK1: at most one kernel (FWD, INV, or CONV with nk = 1): the product is
+// formed in place, which halves the registers held across the inverse
+template <typename T, int R, int TX, bool NK1>
 __global__ void __launch_bounds__(R* TX) k_zreg(ZPassArgs<T> a) {
     using C = cplx<T>;
     constexpr int N = R * R;
@@ -294,11 +296,37 @@ __global__ void __launch_bounds__(R* TX) k_zreg(ZPassArgs<T> a) {
             for (int k = 0; k < R; ++k) out[base + zs * (c + R * k)] = v[k];
         if (a.mode == ZMODE_FWD) return;
     }
+    if (a.kout && ok)
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const long o = base + zs * (c + R * k);
+            a.kout[o] = cscale(v[k], a.kkern[o]);
+        }
+    if constexpr (NK1) {
+        const T* K = a.kern[0];
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[k] = cscale(v[k], ok ? K[base + zs * (c + R * k)] : (T)0);
+        if (a.addin && ok)
+#pragma unroll
+            for (int k = 0; k < R; ++k) v[k] = cadd(v[k], a.addin[base + zs * (c + R * k)]);
+        __syncthreads();
+        rowdata_phase1<C, R>(v, c, lt, R * TX, TX, tw, true);
+        __syncthreads();
+        row_phase<C, R>(v, c, lt, R * TX, TX, true);
+        C* out = a.out[blockIdx.z][0];
+        if (ok)
+#pragma unroll
+            for (int m = 0; m < R; ++m) out[base + zs * (c + R * m)] = v[m];
+        return;
+    }
     for (int q = 0; q < a.nk; ++q) {
         const T* K = a.kern[q];
         C w[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) w[k] = cscale(v[k], ok ? K[base + zs * (c + R * k)] : (T)0);
+        if (q == 0 && a.addin && ok)
+#pragma unroll
+            for (int k = 0; k < R; ++k) w[k] = cadd(w[k], a.addin[base + zs * (c + R * k)]);
         __syncthreads();  // previous readers of the LDS matrix are done
         rowdata_phase1<C, R>(w, c, lt, R * TX, TX, tw, true);
         __syncthreads();
@@ -348,16 +376,25 @@ hipError_t launch_yreg(YPassArgs<T> a, int nz, int nbuf, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <typename T, int R, int TX>
+static hipError_t zreg_go(const ZPassArgs<T>& a, int nbuf, hipStream_t st) {
+    const dim3 grid((a.nxh + TX - 1) / TX, a.ny, nbuf);
+    if (a.mode != ZMODE_CONV || a.nk == 1)
+        hipLaunchKernelGGL((k_zreg<T, R, TX, true>), grid, dim3(R * TX), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_zreg<T, R, TX, false>), grid, dim3(R * TX), 0, st, a);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_zreg(ZPassArgs<T> a, int nbuf, hipStream_t st) {
     const int R = reg_radix(a.ax.n);
     switch (R) {
-    case 4: hipLaunchKernelGGL((k_zreg<T, 4, 32>), dim3((a.nxh + 31) / 32, a.ny, nbuf), dim3(128), 0, st, a); break;
-    case 8: hipLaunchKernelGGL((k_zreg<T, 8, 16>), dim3((a.nxh + 15) / 16, a.ny, nbuf), dim3(128), 0, st, a); break;
-    case 16: hipLaunchKernelGGL((k_zreg<T, 16, 16>), dim3((a.nxh + 15) / 16, a.ny, nbuf), dim3(256), 0, st, a); break;
+    case 4: return zreg_go<T, 4, 32>(a, nbuf, st);
+    case 8: return zreg_go<T, 8, 16>(a, nbuf, st);
+    case 16: return zreg_go<T, 16, 16>(a, nbuf, st);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 template hipError_t launch_xreg<double>(XPassArgs<double>, hipStream_t);

@@ -18,9 +18,9 @@ template <typename T> struct XPassArgs {
     T scale;      // applied after an inverse transform (renor)
     const T* rin;        // R2C: input; CG_S: den; POT: den
     const T* rin2;       // POT: dencg
-    const T* rin3;       // POT: delj4
+    const T* rin3;       // POT: delj4 (null: already in the spectrum)
     const cplx<T>* cin;  // inverse input half spectrum
-    T* rout;             // C2R output; CG_S: dencg; POT: pot4
+    T* rout;             // C2R output; CG_S: dencg; POT: pot4 (null: not stored)
     T* rout2;            // POT: correlation term sto1 (optional)
     cplx<T>* cout;       // forward output half spectrum
     T cp4, cpp4, a0, a1;
@@ -41,6 +41,9 @@ template <typename T> struct ZPassArgs {
     cplx<T>* out[4][3];
     const T* kern[3];
     cplx<T>* fwd_out;  // optional copy of the forward spectrum (CONV)
+    cplx<T>* kout;     // optional kkern * forward spectrum, not inverted (CONV)
+    const T* kkern;
+    const cplx<T>* addin;  // optional z-spectrum added to kern[0] * forward before its inverse (CONV)
 };
 
 template <typename T> hipError_t launch_xpass(XPassArgs<T> a, hipStream_t st);

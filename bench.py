@@ -157,6 +157,43 @@ def run_precision(args, prec, psi, kclass, ev):
             "e0": e0, "e1": e1, "norm": norm, "setup_s": setup_s, "nsteps_total": nsteps_total}
 
 
+def accuracy_run(args, prec, psi):
+    """Untimed: args.check_steps steps from the same state, recording the
+    impurity trajectory every step and the energies at the end."""
+    from hetd import Context, capi
+    n = args.n
+    xm = 0.2 * n
+    p = capi.default_params(nx=n, ny=n, nz=n, xmax=xm, ymax=xm, zmax=xm, npd=13, icon=13, precision=prec)
+    ctx = Context(p, selec="Xe_He", r_cutoff=2.55, umax=2793.9365463)
+    ctx.set_state(psi, (0.0, 0.0, 0.0), (0.0, 0.0, 2.0 * KTOPS))
+    ctx.init()
+    e0 = ctx.energy()
+    traj = ctx.evolve(5e-4 / KTOPS, args.check_steps, traj=True)
+    e1 = ctx.energy()
+    norm = ctx.norm()
+    r, v, _ = ctx.imp()
+    ctx.close()
+    return {"e0": e0, "e1": e1, "norm": norm, "traj": traj, "r": r, "v": v}
+
+
+def compare_accuracy(a64, amx):
+    """Mixed against FP64 after the same steps: relative energy errors (total
+    and per component scale), norm, impurity position / velocity."""
+    import numpy as np
+    out = {"steps": int(len(a64["traj"]))}
+    for k in ("etot", "etot4"):
+        out["dE_over_E_" + k] = abs(amx["e1"][k] - a64["e1"][k]) / abs(a64["e1"][k])
+    for k in ("ekin4", "elj4", "ecor4", "esolid", "ekinx", "eHeX"):
+        out["dE_K_" + k] = amx["e1"][k] - a64["e1"][k]
+    out["dnorm_over_norm"] = abs(amx["norm"] - a64["norm"]) / a64["norm"]
+    out["max_dz_A"] = float(np.max(np.abs(amx["traj"][:, 4] - a64["traj"][:, 4])))
+    out["dr_end_A"] = float(np.linalg.norm(amx["r"] - a64["r"]))
+    out["dv_over_v_end"] = float(np.linalg.norm(amx["v"] - a64["v"]) / np.linalg.norm(a64["v"]))
+    out["energy_drift_fp64"] = (a64["e1"]["etot"] - a64["e0"]["etot"]) / abs(a64["e0"]["etot"])
+    out["energy_drift_mixed"] = (amx["e1"]["etot"] - amx["e0"]["etot"]) / abs(amx["e0"]["etot"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +204,8 @@ def main():
     ap.add_argument("--kernel", default="update", choices=["update", "xpass", "ypass", "zpass", "force"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--check-steps", type=int, default=400,
+                    help="untimed mixed-vs-FP64 accuracy run length (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the ensemble split across "
@@ -186,6 +225,9 @@ def main():
         res["fp64"] = run_precision(args, capi.HETD_FP64, psi, kclass, ev)
     if args.precision in ("both", "mixed"):
         res["mixed"] = run_precision(args, capi.HETD_MIXED, psi, kclass, ev)
+    acc = None
+    if len(res) == 2 and args.check_steps > 0:
+        acc = compare_accuracy(accuracy_run(args, capi.HETD_FP64, psi), accuracy_run(args, capi.HETD_MIXED, psi))
     del psi
     head = "mixed" if "mixed" in res else "fp64"
     r = res[head]
@@ -210,7 +252,7 @@ def main():
                 "dE_over_E_mixed_vs_fp64_etot4": abs(amx["e1"]["etot4"] - a64["e1"]["etot4"]) / abs(a64["e1"]["etot4"]),
                 "dE_over_E_mixed_vs_fp64_etot": abs(amx["e1"]["etot"] - a64["e1"]["etot"]) / abs(a64["e1"]["etot"]),
                 "dnorm_over_norm": abs(amx["norm"] - a64["norm"]) / a64["norm"],
-                "steps_compared": a64["nsteps_total"]}
+                "steps_compared": a64["nsteps_total"], "accuracy_run": acc}
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s_%d.json" % (args.kernel, head, n))
     if os.path.exists(tfile):
