@@ -44,6 +44,10 @@ struct hetd_ctx {
     // delj4 of the last poten: DELJ4_DEN = not formed (make it from den),
     // DELJ4_OK = formed, DELJ4_SPEC = its z spectrum waits in B
     int delj4_state = 0;
+    // force of the current den not formed yet: the next update kernel sums it
+    // (fused), before anything reads the acceleration
+    bool force_pending = false;
+    double* pend_rec = nullptr;  // trajectory row waiting for that acceleration
     float *den32 = nullptr, *pot4_32 = nullptr, *dencg32 = nullptr, *delj4_32 = nullptr, *tmp32 = nullptr;
     float2 *A32 = nullptr, *B32 = nullptr, *C32 = nullptr;
     float *fvlj32 = nullptr, *wcgk32 = nullptr, *kalfs32 = nullptr;
@@ -161,13 +165,23 @@ template <typename T> static hipError_t run_z(hetd_ctx* c, ZPassArgs<T> z, int n
     return e;
 }
 
-static hipError_t run_update(hetd_ctx* c, const UpdateArgs& u, int* nb) {
+static bool force_fusable(const hetd_ctx* c) { return tiled_supported(c->sc) && c->sc.icon != 18; }
+
+static hipError_t run_update(hetd_ctx* c, UpdateArgs& u, int* nb) {
+    u.force = c->force_pending ? 1 : 0;
     double per = 144.0;  // PRED / CORR (FP64 den and pot4)
     if (u.kind == STAGE_RK) per = u.rk_j == 0 ? 96.0 : 80.0;
     // mixed: the FP32 den copy written (+4) balances the FP32 pot4 read (-4)
     tbeg(c, HETD_K_UPDATE);
-    hipError_t e = tiled_supported(c->sc) ? launch_update_tiled(u, c->st, nb) : launch_update(u, c->st, nb);
+    int nbl = 0;
+    hipError_t e = tiled_supported(c->sc) ? launch_update_tiled(u, c->st, &nbl) : launch_update(u, c->st, &nbl);
     tend(c, HETD_K_UPDATE, per * c->n);
+    if (nb) *nb = nbl;
+    if (e != hipSuccess || !u.force) return e;
+    // forceimp of the input state (potenimp.f90:24-55), a = F/mAg_u (DFT4He3d.f90:852)
+    c->force_pending = false;
+    e = launch_force_final(c->imp, c->partial, nbl, 4, 1, c->dxyz, c->mAg, c->pend_rec, c->st);
+    c->pend_rec = nullptr;
     return e;
 }
 
@@ -421,20 +435,31 @@ static int poten_dev(hetd_ctx* c, bool want_ccor = false, bool want_fden = false
 
 // forceimp + aimp = F/mAg_u on the device (potenimp.f90:24-55, DFT4He3d.f90:852)
 static int force_dev(hetd_ctx* c) {
+    c->force_pending = false;
     ForceArgs f{};
     // FP64 den in both modes: the force is a sum with heavy cancellation
     f.g = c->grid; f.sc = c->sc; f.den = c->den; f.imp = c->imp; f.partial = c->partial;
     int nb = 0;
     HCHK(c, run_force(c, f, &nb));
-    HCHK(c, launch_force_final(c->imp, c->partial, nb, c->dxyz, c->mAg, c->st));
+    HCHK(c, launch_force_final(c->imp, c->partial, nb, 3, 0, c->dxyz, c->mAg, c->pend_rec, c->st));
+    c->pend_rec = nullptr;
     return 0;
 }
 
-static int potential_update(hetd_ctx* c) {
+// potenimp (rpot already set) + poten + forceimp.  With defer, the force is
+// left to the next update kernel, which reads the same state.
+static int potential_update(hetd_ctx* c, bool defer) {
     int rc = poten_dev(c, false, false);
     if (rc) return rc;
+    if (defer && force_fusable(c)) {
+        c->force_pending = true;
+        return 0;
+    }
     return force_dev(c);
 }
+
+// an acceleration left to the next update is formed now (end of a call)
+static int settle_force(hetd_ctx* c) { return c->force_pending ? force_dev(c) : 0; }
 
 static UpdateArgs update_base(hetd_ctx* c, double dt) {
     UpdateArgs u{};
@@ -469,7 +494,7 @@ static int steprk_dev(hetd_ctx* c, double dt, bool last_sets_rpot) {
         ImpRkArgs ia{c->imp, j, dt, ar[j], br[j], cr[j], (j <= 2 || last_sets_rpot) ? 1 : 0};
         HCHK(c, launch_imp_rk(ia, c->st));
         if (j <= 2) {
-            int rc = potential_update(c);
+            int rc = potential_update(c, true);
             if (rc) return rc;
         }
     }
@@ -494,7 +519,7 @@ static int steppc_dev(hetd_ctx* c, double dt, bool sets_rpot) {
                  c->ioldv[2] - 1, c->iolda[0] - 1, c->iolda[1] - 1, 0};
     HCHK(c, launch_imp_pred(ia, c->st));
     std::swap(c->iolda[0], c->iolda[1]);
-    int rc = potential_update(c);
+    int rc = potential_update(c, true);
     if (rc) return rc;
     {
         UpdateArgs u = update_base(c, dt);
@@ -504,7 +529,7 @@ static int steppc_dev(hetd_ctx* c, double dt, bool sets_rpot) {
         u.pc = c->pc; u.psi_out = c->spare;
         int nb = 0;
         HCHK(c, run_update(c, u, &nb));
-        HCHK(c, launch_errhe_final(c->imp, c->partial, nb, (double)c->n, c->st));
+        HCHK(c, launch_errhe_final(c->imp, c->partial, nb, 4, (double)c->n, c->st));
         std::swap(c->psi, c->spare);
         const int t = c->ioldp[2];
         c->ioldp[2] = c->ioldp[1];
@@ -762,6 +787,8 @@ int hetd_set_state(hetd_ctx* c, const double* psi, const double r[3], const doub
     HCHK(c, launch_den_from_psi(c->psi, c->den, c->n, c->st));
     if (c->mixed) HCHK(c, launch_den32_from_psi(c->psi, c->den32, c->n, c->st));
     c->delj4_state = DELJ4_DEN;
+    c->force_pending = false;
+    c->pend_rec = nullptr;
     HCHK(c, hipMemsetAsync(c->q, 0, nC, c->st));
     HCHK(c, hipMemsetAsync(c->pc, 0, nC, c->st));
     for (int i = 0; i < 3; ++i) HCHK(c, hipMemsetAsync(c->psiold[i], 0, nC, c->st));
@@ -860,10 +887,15 @@ int hetd_evolve(hetd_ctx* c, double dt, int32_t nsteps, double* traj) {
         }
         if (rc) return rc;
         // driver: potenimp(rimp); poten(); forceimp(rimp,aimp); aimp/=m (DFT4He3d.f90:849-852)
-        rc = potential_update(c);
+        rc = potential_update(c, true);
         if (rc) return rc;
-        if (traj) HCHK(c, launch_traj_record(c->imp, c->traj + (long)s * HETD_TRAJ_STRIDE, c->st));
+        if (traj) {
+            double* row = c->traj + (long)s * HETD_TRAJ_STRIDE;
+            HCHK(c, launch_traj_record(c->imp, row, c->st));
+            if (c->force_pending) c->pend_rec = row + 8;  // aimp, filled when the force is formed
+        }
     }
+    if (int rc = settle_force(c)) return rc;
     if (traj && nsteps > 0)
         HCHK(c, hipMemcpyAsync(traj, c->traj, sizeof(double) * (size_t)nsteps * HETD_TRAJ_STRIDE,
                                hipMemcpyDeviceToHost, c->st));

@@ -20,6 +20,10 @@ template <> struct cplx_t<float> { using type = float2; };
 template <typename T> using cplx = typename cplx_t<T>::type;
 
 template <typename C> __device__ __forceinline__ C cmk(decltype(C::x) a, decltype(C::x) b) { C r; r.x = a; r.y = b; return r; }
+// |z|^2 and |z|: x*x + y*y with one rounding fewer than hypot(x,y)^2 (the
+// reference's abs(z)**2); the two differ by at most an ulp or two
+__device__ __forceinline__ double abs2(double2 v) { return fma(v.x, v.x, v.y * v.y); }
+__device__ __forceinline__ double cabs_(double2 v) { return sqrt(abs2(v)); }
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmk<C>(a.x + b.x, a.y + b.y); }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmk<C>(a.x - b.x, a.y - b.y); }
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) { return cmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }

@@ -106,6 +106,7 @@ struct UpdateArgs {
     double2* pc;
     double* partial;  // CORR: per-block sum |pc|
     int* guard;       // RK: set if den not > 0
+    int force;           // also sum grad(|psi|^2) u_imp (the force of the input state), icon 13/8
     const float* pot4f;  // mixed precision: FP32 mean field (else pot4)
     float* denf;         // mixed precision: FP32 copy of den for the mean field
 };
@@ -175,9 +176,11 @@ hipError_t launch_imp_rk(const ImpRkArgs& a, hipStream_t st);
 hipError_t launch_imp_pred(const ImpPcArgs& a, hipStream_t st);   // sets rpot = predicted R
 hipError_t launch_imp_corr(const ImpPcArgs& a, hipStream_t st);   // set_rpot: rpot = corrected R
 // F = -sum*dxyz, a = F/mass; from reduced sums in imp->F (overwritten)
-hipError_t launch_force_final(ImpState* imp, const double* partial, int nblocks, double dxyz, double mass,
+// F = -sum(partial[b*stride+off+d]) dxyz, a = F/mass (also into rec_a[0..2] if set)
+hipError_t launch_force_final(ImpState* imp, const double* partial, int nblocks, int stride, int off, double dxyz,
+                              double mass, double* rec_a, hipStream_t st);
+hipError_t launch_errhe_final(ImpState* imp, const double* partial, int nblocks, int stride, double npoints,
                               hipStream_t st);
-hipError_t launch_errhe_final(ImpState* imp, const double* partial, int nblocks, double npoints, hipStream_t st);
 hipError_t launch_traj_record(const ImpState* imp, double* rec, hipStream_t st);
 
 // ---------------------------------------------------------- pointwise

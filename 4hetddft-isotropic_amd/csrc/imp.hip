@@ -7,17 +7,18 @@ namespace hetd {
 
 // --------------------------------------------------------- final reductions
 
-// Sums partial[b*ncomp + c] over b in a fixed order: thread t takes
+// Sums partial[b*stride + off + c] over b in a fixed order: thread t takes
 // b = t, t+256, ... then a fixed shuffle tree.  Bitwise reproducible.
 template <int NC>
-__device__ void reduce_fixed(const double* __restrict__ partial, int nblocks, double (&out)[NC]) {
+__device__ void reduce_fixed(const double* __restrict__ partial, int nblocks, double (&out)[NC], int stride = NC,
+                             int off = 0) {
     __shared__ double red[NC * 4];
     double v[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) v[c] = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) v[c] += partial[(long)b * NC + c];
+        for (int c = 0; c < NC; ++c) v[c] += partial[(long)b * stride + off + c];
     block_sum<NC>(v, red);
 #pragma unroll
     for (int c = 0; c < NC; ++c) out[c] = v[c];
@@ -44,35 +45,38 @@ hipError_t launch_reduce_final(const double* partial, int nblocks, int ncomp, do
 
 // F = -sum(grad den * u_imp) * dxyz (potenimp.f90:44-46); aimp = F/mAg_u
 // (DFT4He3d.f90:852)
-__global__ void __launch_bounds__(256) k_force_final(ImpState* imp, const double* partial, int nblocks, double dxyz,
-                                                     double mass) {
+__global__ void __launch_bounds__(256) k_force_final(ImpState* imp, const double* partial, int nblocks, int stride,
+                                                     int off, double dxyz, double mass, double* rec_a) {
     double v[3];
-    reduce_fixed<3>(partial, nblocks, v);
+    reduce_fixed<3>(partial, nblocks, v, stride, off);
     if (threadIdx.x == 0) {
         for (int d = 0; d < 3; ++d) {
             const double F = -v[d] * dxyz;
             imp->F[d] = F;
             imp->a[d] = F / mass;
+            if (rec_a) rec_a[d] = imp->a[d];
         }
     }
 }
 
-hipError_t launch_force_final(ImpState* imp, const double* partial, int nblocks, double dxyz, double mass,
-                              hipStream_t st) {
-    hipLaunchKernelGGL(k_force_final, dim3(1), dim3(256), 0, st, imp, partial, nblocks, dxyz, mass);
+hipError_t launch_force_final(ImpState* imp, const double* partial, int nblocks, int stride, int off, double dxyz,
+                              double mass, double* rec_a, hipStream_t st) {
+    hipLaunchKernelGGL(k_force_final, dim3(1), dim3(256), 0, st, imp, partial, nblocks, stride, off, dxyz, mass,
+                       rec_a);
     return hipGetLastError();
 }
 
 // errHe = c9*sum(abs(pc)) / nxyz (steppc.f90:131,134)
-__global__ void __launch_bounds__(256) k_errhe_final(ImpState* imp, const double* partial, int nblocks, double c9,
-                                                     double n) {
+__global__ void __launch_bounds__(256) k_errhe_final(ImpState* imp, const double* partial, int nblocks, int stride,
+                                                     double c9, double n) {
     double v[1];
-    reduce_fixed<1>(partial, nblocks, v);
+    reduce_fixed<1>(partial, nblocks, v, stride, 0);
     if (threadIdx.x == 0) imp->errHe = (c9 * v[0]) / n;
 }
 
-hipError_t launch_errhe_final(ImpState* imp, const double* partial, int nblocks, double n, hipStream_t st) {
-    hipLaunchKernelGGL(k_errhe_final, dim3(1), dim3(256), 0, st, imp, partial, nblocks, 9.0 / 121.0, n);
+hipError_t launch_errhe_final(ImpState* imp, const double* partial, int nblocks, int stride, double n,
+                              hipStream_t st) {
+    hipLaunchKernelGGL(k_errhe_final, dim3(1), dim3(256), 0, st, imp, partial, nblocks, stride, 9.0 / 121.0, n);
     return hipGetLastError();
 }
 
@@ -192,8 +196,7 @@ static inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 __global__ void k_den_from_psi(const double2* psi, double* den, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        const double a = hypot(psi[i].x, psi[i].y);
-        den[i] = a * a;
+        den[i] = abs2(psi[i]);
     }
 }
 hipError_t launch_den_from_psi(const double2* psi, double* den, long n, hipStream_t st) {
@@ -204,8 +207,7 @@ hipError_t launch_den_from_psi(const double2* psi, double* den, long n, hipStrea
 __global__ void k_den32_from_psi(const double2* psi, float* den, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        const double a = hypot(psi[i].x, psi[i].y);
-        den[i] = (float)(a * a);
+        den[i] = (float)abs2(psi[i]);
     }
 }
 hipError_t launch_den32_from_psi(const double2* psi, float* den, long n, hipStream_t st) {

@@ -50,10 +50,7 @@ __device__ __forceinline__ V stencil_axis(const V* __restrict__ f, long base, in
     return s;
 }
 
-__device__ __forceinline__ double sq_abs(double2 v) {
-    const double a = hypot(v.x, v.y);
-    return a * a;
-}
+__device__ __forceinline__ double sq_abs(double2 v) { return abs2(v); }
 
 // ------------------------------------------------------- fused update
 
@@ -132,7 +129,7 @@ __global__ void __launch_bounds__(TBX* TBY) k_update(UpdateArgs a) {
             a.pc[o] = pcn;
             a.psi_out[o] = pn;
             a.den[o] = sq_abs(pn);
-            pcabs = hypot(pcn.x, pcn.y);
+            pcabs = cabs_(pcn);
         }
     }
     if (a.kind == STAGE_CORR) {
@@ -141,7 +138,7 @@ __global__ void __launch_bounds__(TBX* TBY) k_update(UpdateArgs a) {
         block_sum<1>(v, red);
         if (threadIdx.x == 0) {
             const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
-            a.partial[b] = v[0];
+            a.partial[4 * b] = v[0];  // same layout as the tiled kernel
         }
     }
 }

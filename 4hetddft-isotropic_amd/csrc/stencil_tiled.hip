@@ -69,7 +69,8 @@ __device__ __forceinline__ V load_bc(const V* __restrict__ f, int nx, int ny, in
 enum TKind { TK_UPDATE = 0, TK_FORCE = 1, TK_ENERGY = 2 };
 
 template <int KIND> struct TArgs;
-template <> struct TArgs<TK_UPDATE> { using V = double2; using A = UpdateArgs; static constexpr int NC = 1; };
+// UPDATE partials: [sum |pc| (corrector), force x, y, z (when fused)]
+template <> struct TArgs<TK_UPDATE> { using V = double2; using A = UpdateArgs; static constexpr int NC = 4; };
 template <> struct TArgs<TK_FORCE> { using V = double; using A = ForceArgs; static constexpr int NC = 3; };
 template <> struct TArgs<TK_ENERGY> { using V = double2; using A = EnergyArgs; static constexpr int NC = EN_NCOMP; };
 
@@ -95,9 +96,14 @@ __device__ __forceinline__ V cell_value(const V* __restrict__ f, int nx, int ny,
     return load_bc<V>(f, nx, ny, nz, gx, gy, iz, icon);
 }
 
+__device__ __forceinline__ double dens(double2 v) { return abs2(v); }
+
 // TPY points per thread along y, TNT threads: tile 64 x TTY with
-// TTY = TPY*TNT/64; PF: software-pipelined halo / queue loads
-template <int KIND, int M, int TPY, int TNT, bool PF>
+// TTY = TPY*TNT/64; PF: software-pipelined halo / queue loads.
+// FRC (UPDATE only): the density |psi|^2 of the input state is kept beside
+// psi (LDS plane and z queue) and grad(den) * u_imp summed, which is
+// forceimp (potenimp.f90:24-55) of the state the update starts from.
+template <int KIND, int M, int TPY, int TNT, bool PF, bool FRC>
 __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KIND>::A args) {
     using V = typename TArgs<KIND>::V;
     constexpr int NC = TArgs<KIND>::NC;
@@ -105,6 +111,7 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
     constexpr int TTY = TPY * RS;
     constexpr int LX = TTX + 2 * M, LY = TTY + 2 * M, W = 2 * M + 1;
     __shared__ V lds[LX * LY];
+    __shared__ double ldsd[FRC ? LX * LY : 1];
     __shared__ double red[NC * TNT / 64];
 
     const Grid& g = tc.g;
@@ -143,6 +150,7 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
     int iy[TPY];
     bool valid[TPY];
     V q[TPY][W];
+    double dq[FRC ? TPY : 1][W];  // |psi|^2 of the z queue
 #pragma unroll
     for (int p = 0; p < TPY; ++p) {
         iy[p] = iy0 + ty + RS * p;
@@ -152,6 +160,7 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
             const int zz = iz0 - M + k;
             q[p][k] = (valid[p] && (zok || (zz >= 0 && zz < nz))) ? load_bc<V>(f, nx, ny, nz, ix, iy[p], zz, icon)
                                                                    : vz<V>::zero();
+            if constexpr (FRC) dq[p][k] = dens(q[p][k]);
         }
     }
 
@@ -205,16 +214,24 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                                                         : vz<V>::zero();
             }
             q[p][W - 1] = qn[p];
+            if constexpr (FRC) dq[p][W - 1] = dens(qn[p]);
         }
         __syncthreads();
         // plane iz into LDS: interior from the queues, halo from the prefetch
 #pragma unroll
-        for (int p = 0; p < TPY; ++p)
-            lds[(ty + RS * p + M) * LX + tx + M] =
-                valid[p] ? q[p][M] : cell_value<V, M>(f, nx, ny, nz, ix, iy[p], iz, icon);
+        for (int p = 0; p < TPY; ++p) {
+            const int li = (ty + RS * p + M) * LX + tx + M;
+            const V v = valid[p] ? q[p][M] : cell_value<V, M>(f, nx, ny, nz, ix, iy[p], iz, icon);
+            lds[li] = v;
+            if constexpr (FRC) ldsd[li] = valid[p] ? dq[p][M] : dens(v);
+        }
 #pragma unroll
         for (int k = 0; k < KH; ++k)
-            if (hl[k] >= 0) lds[hl[k]] = PF ? hreg[k] : cell_value<V, M>(f, nx, ny, nz, hgx[k], hgy[k], iz, icon);
+            if (hl[k] >= 0) {
+                const V v = PF ? hreg[k] : cell_value<V, M>(f, nx, ny, nz, hgx[k], hgy[k], iz, icon);
+                lds[hl[k]] = v;
+                if constexpr (FRC) ldsd[hl[k]] = dens(v);
+            }
         __syncthreads();
         if (PF && iz + 1 < izend) {
 #pragma unroll
@@ -247,6 +264,24 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
             const double u = uimp_at(g.potion, g.npot, g.delinter, g.x[ix] - R[0], dy * dy + dz * dz);
             if constexpr (KIND == TK_UPDATE) {
                 const UpdateArgs& a = args;
+                if constexpr (FRC) {
+                    const double* e1 = tc.sc.c1[0];
+                    const double* e2 = tc.sc.c1[1];
+                    const double* e3 = tc.sc.c1[2];
+                    double gx = 0.0, gy = 0.0, gz = 0.0;
+                    if (nx >= W)
+#pragma unroll
+                        for (int j = 0; j < W; ++j) gx = fma(e1[j], ldsd[ly * LX + tx + j], gx);
+                    if (ny >= W)
+#pragma unroll
+                        for (int j = 0; j < W; ++j) gy = fma(e2[j], ldsd[(ly - M + j) * LX + tx + M], gy);
+                    if (nz >= W)
+#pragma unroll
+                        for (int j = 0; j < W; ++j) gz = fma(e3[j], dq[p][j], gz);
+                    acc[1] += gx * u;
+                    acc[2] += gy * u;
+                    acc[3] += gz * u;
+                }
                 const double lr = (sx.x + sy.x) + sz.x, li = (sx.y + sy.y) + sz.y;
                 const double2 ps = q[p][M];
                 const double pot = a.pot4f ? (double)a.pot4f[o] : a.pot4[o];
@@ -268,7 +303,7 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     if (a.rk_j == 0) a.hnew[o] = fv;
                     pn.x = ps.x + a.dt * s1.x;
                     pn.y = ps.y + a.dt * s1.y;
-                    const double h = hypot(pn.x, pn.y), d = h * h;
+                    const double d = abs2(pn);
                     a.psi_out[o] = pn;
                     if (a.denf) a.denf[o] = (float)d;
                     a.den[o] = d;
@@ -285,7 +320,7 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     a.psi_out[o] = pn;
                     a.pc[o] = P;
                     a.hnew[o] = fv;
-                    const double h = hypot(pn.x, pn.y), d = h * h;
+                    const double d = abs2(pn);
                     if (a.denf) a.denf[o] = (float)d;
                     a.den[o] = d;
                 } else {
@@ -301,10 +336,10 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     pn.y = Cc.y + c9 * pcn.y;
                     a.pc[o] = pcn;
                     a.psi_out[o] = pn;
-                    const double h = hypot(pn.x, pn.y), d = h * h;
+                    const double d = abs2(pn);
                     if (a.denf) a.denf[o] = (float)d;
                     a.den[o] = d;
-                    acc[0] += hypot(pcn.x, pcn.y);
+                    acc[0] += cabs_(pcn);
                 }
             } else if constexpr (KIND == TK_FORCE) {
                 acc[0] += (double)sx * u;
@@ -312,10 +347,9 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                 acc[2] += (double)sz * u;
             } else {
                 const EnergyArgs& a = args;
-                const double h1 = hypot(sx.x, sx.y), h2 = hypot(sy.x, sy.y), h3 = hypot(sz.x, sz.y);
-                acc[EN_KIN] += (h1 * h1 + h2 * h2) + h3 * h3;
+                acc[EN_KIN] += (abs2(sx) + abs2(sy)) + abs2(sz);
                 // den = |psi|^2 of the FP64 state (equal to the den array in FP64)
-                const double hp = hypot(q[p][M].x, q[p][M].y), d = hp * hp;
+                const double d = abs2(q[p][M]);
                 const double cg = a.dencgf ? (double)a.dencgf[o] : a.dencg[o];
                 acc[EN_LJ] += (a.delj4f ? (double)a.delj4f[o] : a.delj4[o]) * d;
                 acc[EN_COR] += d * cg * cg * (a.aux1 + a.aux2 * cg);
@@ -331,10 +365,13 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
 #pragma unroll
         for (int p = 0; p < TPY; ++p)
 #pragma unroll
-            for (int k = 0; k < W - 1; ++k) q[p][k] = q[p][k + 1];
+            for (int k = 0; k < W - 1; ++k) {
+                q[p][k] = q[p][k + 1];
+                if constexpr (FRC) dq[p][k] = dq[p][k + 1];
+            }
     }
     bool reduce = true;
-    if constexpr (KIND == TK_UPDATE) reduce = args.kind == STAGE_CORR;
+    if constexpr (KIND == TK_UPDATE) reduce = FRC || args.kind == STAGE_CORR;
     if (reduce) {
         block_sum<NC>(acc, red);
         if (threadIdx.x == 0) {
@@ -366,25 +403,36 @@ static TiledCommon tiled_common(const Grid& g, const StencilCoef& sc, int tty) {
 
 bool tiled_supported(const StencilCoef& sc) { return sc.m == 4 || sc.m == 6; }
 
-// tile variants (points per thread, threads, pipelined loads).  Variant 2
-// (one point per thread, 512 threads, 64x8 tile) was fastest for all three
-// kinds on MI355X at 256^3: its register queue leaves room for 3-4 waves per
-// SIMD, where two points per thread cap the complex kinds at 1-2.
-// HETD_TILE_<KIND>=0..3 selects another variant.
+// tile variants (points per thread, threads, pipelined loads).  Measured on
+// MI355X at 256^3: one point per thread wins for every kind (the complex
+// register queue of two points caps a SIMD at 1-2 waves); for UPDATE, 256
+// threads (64x4 tiles, several workgroups per CU, so one workgroup's loads
+// overlap another's arithmetic) with the next plane's loads issued before
+// the current plane is computed beat 512-thread 64x8 tiles by 0.15 ms/step.
+// HETD_TILE_<KIND>=0..6 selects another variant.
 struct TileVariant { int tpy, tnt; bool pf; };
-static constexpr TileVariant kVariants[4] = {{2, 256, false}, {2, 256, true}, {1, 512, false}, {4, 256, true}};
+static constexpr int kNVariants = 7;
+static constexpr TileVariant kVariants[kNVariants] = {{2, 256, false}, {2, 256, true}, {1, 512, false}, {4, 256, true},
+                                                     {1, 256, false}, {1, 512, true},  {1, 256, true}};
 
 static int variant_for(int kind) {
     static const char* names[3] = {"HETD_TILE_UPDATE", "HETD_TILE_FORCE", "HETD_TILE_ENERGY"};
     const char* e = getenv(names[kind]);
-    if (e && e[0] >= '0' && e[0] <= '3') return e[0] - '0';
-    return 2;
+    if (e && e[0] >= '0' && e[0] < '0' + kNVariants) return e[0] - '0';
+    return kind == TK_UPDATE ? 6 : 2;
 }
 
 template <int KIND, int M, int V>
 static void launch_v(const TiledCommon& tc, const typename TArgs<KIND>::A& a, int nlaunch, hipStream_t st) {
     constexpr TileVariant tv = kVariants[V];
-    hipLaunchKernelGGL((k_tiled<KIND, M, tv.tpy, tv.tnt, tv.pf>), dim3(nlaunch), dim3(tv.tnt), 0, st, tc, a);
+    if constexpr (KIND == TK_UPDATE) {
+        if (a.force) {
+            hipLaunchKernelGGL((k_tiled<KIND, M, tv.tpy, tv.tnt, tv.pf, true>), dim3(nlaunch), dim3(tv.tnt), 0, st,
+                               tc, a);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_tiled<KIND, M, tv.tpy, tv.tnt, tv.pf, false>), dim3(nlaunch), dim3(tv.tnt), 0, st, tc, a);
 }
 
 template <int KIND, int M>
@@ -393,7 +441,10 @@ static void launch_m(int v, const TiledCommon& tc, const typename TArgs<KIND>::A
     case 0: launch_v<KIND, M, 0>(tc, a, nlaunch, st); break;
     case 1: launch_v<KIND, M, 1>(tc, a, nlaunch, st); break;
     case 2: launch_v<KIND, M, 2>(tc, a, nlaunch, st); break;
-    default: launch_v<KIND, M, 3>(tc, a, nlaunch, st); break;
+    case 3: launch_v<KIND, M, 3>(tc, a, nlaunch, st); break;
+    case 4: launch_v<KIND, M, 4>(tc, a, nlaunch, st); break;
+    case 5: launch_v<KIND, M, 5>(tc, a, nlaunch, st); break;
+    default: launch_v<KIND, M, 6>(tc, a, nlaunch, st); break;
     }
 }
 
