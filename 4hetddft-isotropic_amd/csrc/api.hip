@@ -47,6 +47,7 @@ struct hetd_ctx {
     // force of the current den not formed yet: the next update kernel sums it
     // (fused), before anything reads the acceleration
     bool force_pending = false;
+    bool den64_ok = true;  // mixed: the FP64 den array matches psi (updates write only FP32 den)
     double* pend_rec = nullptr;  // trajectory row waiting for that acceleration
     float *den32 = nullptr, *pot4_32 = nullptr, *dencg32 = nullptr, *delj4_32 = nullptr, *tmp32 = nullptr;
     float2 *A32 = nullptr, *B32 = nullptr, *C32 = nullptr;
@@ -171,7 +172,12 @@ static hipError_t run_update(hetd_ctx* c, UpdateArgs& u, int* nb) {
     u.force = c->force_pending ? 1 : 0;
     double per = 144.0;  // PRED / CORR (FP64 den and pot4)
     if (u.kind == STAGE_RK) per = u.rk_j == 0 ? 96.0 : 80.0;
-    // mixed: the FP32 den copy written (+4) balances the FP32 pot4 read (-4)
+    if (u.denf) {
+        c->den64_ok = false;  // only the FP32 den is written
+        per -= 4.0 + (u.kind == STAGE_RK ? 0.0 : 4.0);  // FP32 den written, FP32 pot4 read
+        // complex64 history: PRED reads 2 and writes 1, CORR reads 2, RK stage 0 writes 1
+        per -= u.kind == STAGE_PRED ? 24.0 : u.kind == STAGE_CORR ? 16.0 : (u.rk_j == 0 ? 8.0 : 0.0);
+    }
     tbeg(c, HETD_K_UPDATE);
     int nbl = 0;
     hipError_t e = tiled_supported(c->sc) ? launch_update_tiled(u, c->st, &nbl) : launch_update(u, c->st, &nbl);
@@ -434,8 +440,17 @@ static int poten_dev(hetd_ctx* c, bool want_ccor = false, bool want_fden = false
 }
 
 // forceimp + aimp = F/mAg_u on the device (potenimp.f90:24-55, DFT4He3d.f90:852)
+// FP64 den from psi where the mixed path has written only the FP32 copy
+static int ensure_den64(hetd_ctx* c) {
+    if (c->den64_ok) return 0;
+    HCHK(c, launch_den_from_psi(c->psi, c->den, c->n, c->st));
+    c->den64_ok = true;
+    return 0;
+}
+
 static int force_dev(hetd_ctx* c) {
     c->force_pending = false;
+    if (int rc = ensure_den64(c)) return rc;
     ForceArgs f{};
     // FP64 den in both modes: the force is a sum with heavy cancellation
     f.g = c->grid; f.sc = c->sc; f.den = c->den; f.imp = c->imp; f.partial = c->partial;
@@ -667,7 +682,9 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
     double w1[kMaxNpd], w2[kMaxNpd];
     setup::fd_weights(p->npd, 1, w1);
     setup::fd_weights(p->npd, 2, w2);
+    for (int j = 0; j < p->npd; ++j) { c->sc.w1[j] = w1[j]; c->sc.w2[j] = w2[j]; }
     for (int a = 0; a < 3; ++a) {
+        c->sc.h[a] = hh[a];
         const double s1 = 1.0 / hh[a], s2 = 1.0 / (hh[a] * hh[a]);
         for (int j = 0; j < p->npd; ++j) {
             c->sc.c1[a][j] = w1[j] * s1;
@@ -789,6 +806,7 @@ int hetd_set_state(hetd_ctx* c, const double* psi, const double r[3], const doub
     c->delj4_state = DELJ4_DEN;
     c->force_pending = false;
     c->pend_rec = nullptr;
+    c->den64_ok = true;
     HCHK(c, hipMemsetAsync(c->q, 0, nC, c->st));
     HCHK(c, hipMemsetAsync(c->pc, 0, nC, c->st));
     for (int i = 0; i < 3; ++i) HCHK(c, hipMemsetAsync(c->psiold[i], 0, nC, c->st));
@@ -975,6 +993,7 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
     size_t bytes = nR;
     int rc = 0;
     const bool mx = c->mixed;
+    if ((rc = ensure_den64(c))) return rc;
     switch (which) {
     case HETD_F_PSI: src = c->psi; bytes = nC; break;
     case HETD_F_DEN: src = c->den; break;

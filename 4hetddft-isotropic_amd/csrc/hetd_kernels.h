@@ -60,8 +60,10 @@ template <typename T> hipError_t launch_zreg(ZPassArgs<T> a, int nbuf, hipStream
 // Stencil weights already divided by h**id (init_deriv_parallel.f90:415-431).
 struct StencilCoef {
     int npd, m, icon;
-    double c1[3][kMaxNpd];  // first derivative per axis
-    double c2[3][kMaxNpd];  // second derivative per axis
+    double c1[3][kMaxNpd];  // first derivative per axis (weights / h)
+    double c2[3][kMaxNpd];  // second derivative per axis (weights / h^2)
+    double w1[kMaxNpd], w2[kMaxNpd];  // the weights themselves (centred, symmetric / antisymmetric)
+    double h[3];
 };
 
 struct Grid {
@@ -107,8 +109,10 @@ struct UpdateArgs {
     double* partial;  // CORR: per-block sum |pc|
     int* guard;       // RK: set if den not > 0
     int force;           // also sum grad(|psi|^2) u_imp (the force of the input state), icon 13/8
-    const float* pot4f;  // mixed precision: FP32 mean field (else pot4)
-    float* denf;         // mixed precision: FP32 copy of den for the mean field
+    // mixed precision when denf is set: FP32 mean field pot4f in, FP32 den out
+    // (no FP64 den), and hnew/h1/h2 hold complex64 RHS history
+    const float* pot4f;
+    float* denf;
 };
 
 hipError_t launch_update(const UpdateArgs& a, hipStream_t st, int* nblocks_out);

@@ -22,18 +22,17 @@ namespace hetd {
 constexpr int TTX = 64;
 
 template <typename V> struct vz;
-template <> struct vz<float> {
-    static __device__ __forceinline__ float zero() { return 0.0f; }
-    static __device__ __forceinline__ float scale(float v, double s) { return v * (float)s; }
-    static __device__ __forceinline__ void fma_(float& s, double c, float v) { s = fmaf((float)c, v, s); }
-};
 template <> struct vz<double> {
     static __device__ __forceinline__ double zero() { return 0.0; }
+    static __device__ __forceinline__ double add(double a, double b) { return a + b; }
+    static __device__ __forceinline__ double sub(double a, double b) { return a - b; }
     static __device__ __forceinline__ double scale(double v, double s) { return v * s; }
     static __device__ __forceinline__ void fma_(double& s, double c, double v) { s = fma(c, v, s); }
 };
 template <> struct vz<double2> {
     static __device__ __forceinline__ double2 zero() { return make_double2(0.0, 0.0); }
+    static __device__ __forceinline__ double2 add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+    static __device__ __forceinline__ double2 sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
     static __device__ __forceinline__ double2 scale(double2 v, double s) { return make_double2(v.x * s, v.y * s); }
     static __device__ __forceinline__ void fma_(double2& s, double c, double2 v) {
         s.x = fma(c, v.x, s.x);
@@ -74,11 +73,20 @@ template <> struct TArgs<TK_UPDATE> { using V = double2; using A = UpdateArgs; s
 template <> struct TArgs<TK_FORCE> { using V = double; using A = ForceArgs; static constexpr int NC = 3; };
 template <> struct TArgs<TK_ENERGY> { using V = double2; using A = EnergyArgs; static constexpr int NC = EN_NCOMP; };
 
+// Stencils in folded form: the centred weights are symmetric (second
+// derivative) or antisymmetric (first), so
+//   d2 f = s2 * (w2h[0] f_0 + sum_k w2h[k] (f_k + f_-k)),
+//   d1 f = s1 * sum_k w1h[k] (f_k - f_-k),
+// with one set of weights for all axes and s1 = 1/h, s2 = 1/h^2 per axis.
+// (The reference applies weights/h^id term by term; the two differ only in
+// rounding.)  Few distinct constants keep the kernel's scalar registers free.
 struct TiledCommon {
     Grid g;
-    StencilCoef sc;
+    int icon;
     int zc;          // z planes per workgroup
     int ntx, nty, ntz;
+    double w1h[kMaxNpd / 2 + 1], w2h[kMaxNpd / 2 + 1];
+    double s1[3], s2[3];
 };
 
 // value of f at grid column (gx, gy), plane iz (in range), for an LDS cell:
@@ -98,12 +106,31 @@ __device__ __forceinline__ V cell_value(const V* __restrict__ f, int nx, int ny,
 
 __device__ __forceinline__ double dens(double2 v) { return abs2(v); }
 
+// RHS history hpsiold: complex128, or complex64 in mixed precision
+template <bool MIX> __device__ __forceinline__ double2 get_hist(const double2* h, long o) {
+    if constexpr (MIX) {
+        const float2 v = reinterpret_cast<const float2*>(h)[o];
+        return make_double2((double)v.x, (double)v.y);
+    } else {
+        return h[o];
+    }
+}
+template <bool MIX> __device__ __forceinline__ void put_hist(double2* h, long o, double2 v) {
+    if constexpr (MIX) reinterpret_cast<float2*>(h)[o] = make_float2((float)v.x, (float)v.y);
+    else h[o] = v;
+}
+template <bool MIX> __device__ __forceinline__ void put_den(const UpdateArgs& a, long o, double d) {
+    if constexpr (MIX) a.denf[o] = (float)d;
+    else a.den[o] = d;
+}
+
 // TPY points per thread along y, TNT threads: tile 64 x TTY with
 // TTY = TPY*TNT/64; PF: software-pipelined halo / queue loads.
 // FRC (UPDATE only): the density |psi|^2 of the input state is kept beside
 // psi (LDS plane and z queue) and grad(den) * u_imp summed, which is
 // forceimp (potenimp.f90:24-55) of the state the update starts from.
-template <int KIND, int M, int TPY, int TNT, bool PF, bool FRC>
+// MIX (UPDATE only): FP32 mean field in, FP32 den and complex64 RHS history.
+template <int KIND, int M, int TPY, int TNT, bool PF, bool FRC, bool MIX>
 __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KIND>::A args) {
     using V = typename TArgs<KIND>::V;
     constexpr int NC = TArgs<KIND>::NC;
@@ -115,7 +142,7 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
     __shared__ double red[NC * TNT / 64];
 
     const Grid& g = tc.g;
-    const int nx = g.nx, ny = g.ny, nz = g.nz, icon = tc.sc.icon;
+    const int nx = g.nx, ny = g.ny, nz = g.nz, icon = tc.icon;
     // XCD-aware tile order: the grid is padded to 8*per blocks and the blocks
     // dealt to one XCD (b % 8 equal) take a contiguous range of per tiles
     const int nb = tc.ntx * tc.nty * tc.ntz;
@@ -129,19 +156,8 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
     const int tx = threadIdx.x % TTX, ty = threadIdx.x / TTX;
     const int ix = ix0 + tx;
     const V* __restrict__ f;
-    const double* c1;
-    const double* c2;
-    const double* c3;
-    if constexpr (KIND == TK_UPDATE) {
-        f = args.psi;
-        c1 = tc.sc.c2[0]; c2 = tc.sc.c2[1]; c3 = tc.sc.c2[2];
-    } else if constexpr (KIND == TK_FORCE) {
-        f = args.den;
-        c1 = tc.sc.c1[0]; c2 = tc.sc.c1[1]; c3 = tc.sc.c1[2];
-    } else {
-        f = args.psi;
-        c1 = tc.sc.c1[0]; c2 = tc.sc.c1[1]; c3 = tc.sc.c1[2];
-    }
+    if constexpr (KIND == TK_FORCE) f = args.den;
+    else f = args.psi;
     const bool zok = nz >= W;  // one periodic wrap reaches every z neighbour
     double acc[NC];
 #pragma unroll
@@ -249,15 +265,47 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
             if (!valid[p]) continue;
             const int ly = ty + RS * p + M;
             V sx = vz<V>::zero(), sy = vz<V>::zero(), sz = vz<V>::zero();
-            if (nx >= 2 * M + 1)
+            const int cx = ly * LX + tx + M;  // LDS index of the point
+            if constexpr (KIND == TK_UPDATE) {
+                // second derivative (Laplacian) of psi
+                if (nx >= W) {
+                    V t = vz<V>::scale(lds[cx], tc.w2h[0]);
 #pragma unroll
-                for (int j = 0; j < W; ++j) vz<V>::fma_(sx, c1[j], lds[ly * LX + tx + j]);
-            if (ny >= 2 * M + 1)
+                    for (int k = 1; k <= M; ++k) vz<V>::fma_(t, tc.w2h[k], vz<V>::add(lds[cx + k], lds[cx - k]));
+                    sx = vz<V>::scale(t, tc.s2[0]);
+                }
+                if (ny >= W) {
+                    V t = vz<V>::scale(lds[cx], tc.w2h[0]);
 #pragma unroll
-                for (int j = 0; j < W; ++j) vz<V>::fma_(sy, c2[j], lds[(ly - M + j) * LX + tx + M]);
-            if (nz >= 2 * M + 1)
+                    for (int k = 1; k <= M; ++k)
+                        vz<V>::fma_(t, tc.w2h[k], vz<V>::add(lds[cx + k * LX], lds[cx - k * LX]));
+                    sy = vz<V>::scale(t, tc.s2[1]);
+                }
+                if (nz >= W) {
+                    V t = vz<V>::scale(q[p][M], tc.w2h[0]);
 #pragma unroll
-                for (int j = 0; j < W; ++j) vz<V>::fma_(sz, c3[j], q[p][j]);
+                    for (int k = 1; k <= M; ++k) vz<V>::fma_(t, tc.w2h[k], vz<V>::add(q[p][M + k], q[p][M - k]));
+                    sz = vz<V>::scale(t, tc.s2[2]);
+                }
+            } else {
+                // gradient of den (FORCE) or psi (ENERGY)
+                if (nx >= W) {
+#pragma unroll
+                    for (int k = 1; k <= M; ++k) vz<V>::fma_(sx, tc.w1h[k], vz<V>::sub(lds[cx + k], lds[cx - k]));
+                    sx = vz<V>::scale(sx, tc.s1[0]);
+                }
+                if (ny >= W) {
+#pragma unroll
+                    for (int k = 1; k <= M; ++k)
+                        vz<V>::fma_(sy, tc.w1h[k], vz<V>::sub(lds[cx + k * LX], lds[cx - k * LX]));
+                    sy = vz<V>::scale(sy, tc.s1[1]);
+                }
+                if (nz >= W) {
+#pragma unroll
+                    for (int k = 1; k <= M; ++k) vz<V>::fma_(sz, tc.w1h[k], vz<V>::sub(q[p][M + k], q[p][M - k]));
+                    sz = vz<V>::scale(sz, tc.s1[2]);
+                }
+            }
             const long o = (long)ix + (long)nx * ((long)iy[p] + (long)ny * iz);
             const double* R = args.imp->rpot;
             const double dz = g.z[iz] - R[2], dy = g.y[iy[p]] - R[1];
@@ -265,26 +313,32 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
             if constexpr (KIND == TK_UPDATE) {
                 const UpdateArgs& a = args;
                 if constexpr (FRC) {
-                    const double* e1 = tc.sc.c1[0];
-                    const double* e2 = tc.sc.c1[1];
-                    const double* e3 = tc.sc.c1[2];
+                    // grad den of the input state, as the FORCE kind forms it
                     double gx = 0.0, gy = 0.0, gz = 0.0;
-                    if (nx >= W)
+                    if (nx >= W) {
 #pragma unroll
-                        for (int j = 0; j < W; ++j) gx = fma(e1[j], ldsd[ly * LX + tx + j], gx);
-                    if (ny >= W)
+                        for (int k = 1; k <= M; ++k) gx = fma(tc.w1h[k], ldsd[cx + k] - ldsd[cx - k], gx);
+                        gx = gx * tc.s1[0];
+                    }
+                    if (ny >= W) {
 #pragma unroll
-                        for (int j = 0; j < W; ++j) gy = fma(e2[j], ldsd[(ly - M + j) * LX + tx + M], gy);
-                    if (nz >= W)
+                        for (int k = 1; k <= M; ++k) gy = fma(tc.w1h[k], ldsd[cx + k * LX] - ldsd[cx - k * LX], gy);
+                        gy = gy * tc.s1[1];
+                    }
+                    if (nz >= W) {
 #pragma unroll
-                        for (int j = 0; j < W; ++j) gz = fma(e3[j], dq[p][j], gz);
+                        for (int k = 1; k <= M; ++k) gz = fma(tc.w1h[k], dq[p][M + k] - dq[p][M - k], gz);
+                        gz = gz * tc.s1[2];
+                    }
                     acc[1] += gx * u;
                     acc[2] += gy * u;
                     acc[3] += gz * u;
                 }
                 const double lr = (sx.x + sy.x) + sz.x, li = (sx.y + sy.y) + sz.y;
                 const double2 ps = q[p][M];
-                const double pot = a.pot4f ? (double)a.pot4f[o] : a.pot4[o];
+                double pot;
+                if constexpr (MIX) pot = (double)a.pot4f[o];
+                else pot = a.pot4[o];
                 const double tr = g.lambdah * ((((((1.0 + g.tcx[ix]) + 1.0) + g.tcy[iy[p]]) + 1.0) + g.tcz[iz]));
                 const double ar = lr * a.h2o2m4 - pot * ps.x;
                 const double ai = li * a.h2o2m4 - pot * ps.y;
@@ -300,17 +354,17 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     qn.x = (qq.x + 3.0 * s1.x) - a.rc * fv.x;
                     qn.y = (qq.y + 3.0 * s1.y) - a.rc * fv.y;
                     a.q[o] = qn;
-                    if (a.rk_j == 0) a.hnew[o] = fv;
+                    if (a.rk_j == 0) put_hist<MIX>(a.hnew, o, fv);
                     pn.x = ps.x + a.dt * s1.x;
                     pn.y = ps.y + a.dt * s1.y;
                     const double d = abs2(pn);
                     a.psi_out[o] = pn;
-                    if (a.denf) a.denf[o] = (float)d;
-                    a.den[o] = d;
+                    put_den<MIX>(a, o, d);
                     if (!(d > 0.0)) atomicOr(a.guard, 1);
                 } else if (a.kind == STAGE_PRED) {
                     const double c4 = (4.0 / 3.0) * a.dt;
-                    const double2 p3 = a.po3[o], h1 = a.h1[o], h2 = a.h2[o], pc = a.pc[o];
+                    const double2 p3 = a.po3[o], h1 = get_hist<MIX>(a.h1, o), h2 = get_hist<MIX>(a.h2, o);
+                    const double2 pc = a.pc[o];
                     double2 P;
                     P.x = p3.x + c4 * ((2.0 * fv.x - h1.x) + 2.0 * h2.x);
                     P.y = p3.y + c4 * ((2.0 * fv.y - h1.y) + 2.0 * h2.y);
@@ -319,12 +373,12 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     pn.y = P.y - c112 * pc.y;
                     a.psi_out[o] = pn;
                     a.pc[o] = P;
-                    a.hnew[o] = fv;
+                    put_hist<MIX>(a.hnew, o, fv);
                     const double d = abs2(pn);
-                    if (a.denf) a.denf[o] = (float)d;
-                    a.den[o] = d;
+                    put_den<MIX>(a, o, d);
                 } else {
-                    const double2 p3 = a.po3[o], p2 = a.po2[o], h1 = a.h1[o], h2 = a.h2[o], pc = a.pc[o];
+                    const double2 p3 = a.po3[o], p2 = a.po2[o], pc = a.pc[o];
+                    const double2 h1 = get_hist<MIX>(a.h1, o), h2 = get_hist<MIX>(a.h2, o);
                     const double t3 = 3.0 * a.dt;
                     double2 Cc, pcn;
                     Cc.x = 0.125 * ((9.0 * p3.x - p2.x) + t3 * ((fv.x + 2.0 * h1.x) - h2.x));
@@ -337,8 +391,7 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     a.pc[o] = pcn;
                     a.psi_out[o] = pn;
                     const double d = abs2(pn);
-                    if (a.denf) a.denf[o] = (float)d;
-                    a.den[o] = d;
+                    put_den<MIX>(a, o, d);
                     acc[0] += cabs_(pcn);
                 }
             } else if constexpr (KIND == TK_FORCE) {
@@ -386,7 +439,15 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
 static TiledCommon tiled_common(const Grid& g, const StencilCoef& sc, int tty) {
     TiledCommon tc;
     tc.g = g;
-    tc.sc = sc;
+    tc.icon = sc.icon;
+    for (int k = 0; k <= sc.m; ++k) {
+        tc.w1h[k] = sc.w1[sc.m + k];
+        tc.w2h[k] = sc.w2[sc.m + k];
+    }
+    for (int a = 0; a < 3; ++a) {
+        tc.s1[a] = 1.0 / sc.h[a];
+        tc.s2[a] = 1.0 / (sc.h[a] * sc.h[a]);
+    }
     tc.ntx = (g.nx + TTX - 1) / TTX;
     tc.nty = (g.ny + tty - 1) / tty;
     const long nbxy = (long)tc.ntx * tc.nty;
@@ -409,30 +470,39 @@ bool tiled_supported(const StencilCoef& sc) { return sc.m == 4 || sc.m == 6; }
 // threads (64x4 tiles, several workgroups per CU, so one workgroup's loads
 // overlap another's arithmetic) with the next plane's loads issued before
 // the current plane is computed beat 512-thread 64x8 tiles by 0.15 ms/step.
-// HETD_TILE_<KIND>=0..6 selects another variant.
+// HETD_TILE_<KIND>=0..3 selects another variant.
 struct TileVariant { int tpy, tnt; bool pf; };
-static constexpr int kNVariants = 7;
-static constexpr TileVariant kVariants[kNVariants] = {{2, 256, false}, {2, 256, true}, {1, 512, false}, {4, 256, true},
-                                                     {1, 256, false}, {1, 512, true},  {1, 256, true}};
+static constexpr int kNVariants = 4;
+static constexpr TileVariant kVariants[kNVariants] = {{1, 512, false}, {1, 256, false}, {1, 512, true}, {1, 256, true}};
 
 static int variant_for(int kind) {
     static const char* names[3] = {"HETD_TILE_UPDATE", "HETD_TILE_FORCE", "HETD_TILE_ENERGY"};
     const char* e = getenv(names[kind]);
     if (e && e[0] >= '0' && e[0] < '0' + kNVariants) return e[0] - '0';
-    return kind == TK_UPDATE ? 6 : 2;
+    return kind == TK_UPDATE ? 3 : 0;
+}
+
+template <int KIND, int M, int V, bool FRC, bool MIX>
+static void launch_k(const TiledCommon& tc, const typename TArgs<KIND>::A& a, int nlaunch, hipStream_t st) {
+    constexpr TileVariant tv = kVariants[V];
+    hipLaunchKernelGGL((k_tiled<KIND, M, tv.tpy, tv.tnt, tv.pf, FRC, MIX>), dim3(nlaunch), dim3(tv.tnt), 0, st, tc,
+                       a);
 }
 
 template <int KIND, int M, int V>
 static void launch_v(const TiledCommon& tc, const typename TArgs<KIND>::A& a, int nlaunch, hipStream_t st) {
-    constexpr TileVariant tv = kVariants[V];
     if constexpr (KIND == TK_UPDATE) {
+        const bool mix = a.denf != nullptr;
         if (a.force) {
-            hipLaunchKernelGGL((k_tiled<KIND, M, tv.tpy, tv.tnt, tv.pf, true>), dim3(nlaunch), dim3(tv.tnt), 0, st,
-                               tc, a);
-            return;
+            if (mix) launch_k<KIND, M, V, true, true>(tc, a, nlaunch, st);
+            else launch_k<KIND, M, V, true, false>(tc, a, nlaunch, st);
+        } else {
+            if (mix) launch_k<KIND, M, V, false, true>(tc, a, nlaunch, st);
+            else launch_k<KIND, M, V, false, false>(tc, a, nlaunch, st);
         }
+    } else {
+        launch_k<KIND, M, V, false, false>(tc, a, nlaunch, st);
     }
-    hipLaunchKernelGGL((k_tiled<KIND, M, tv.tpy, tv.tnt, tv.pf, false>), dim3(nlaunch), dim3(tv.tnt), 0, st, tc, a);
 }
 
 template <int KIND, int M>
@@ -441,10 +511,7 @@ static void launch_m(int v, const TiledCommon& tc, const typename TArgs<KIND>::A
     case 0: launch_v<KIND, M, 0>(tc, a, nlaunch, st); break;
     case 1: launch_v<KIND, M, 1>(tc, a, nlaunch, st); break;
     case 2: launch_v<KIND, M, 2>(tc, a, nlaunch, st); break;
-    case 3: launch_v<KIND, M, 3>(tc, a, nlaunch, st); break;
-    case 4: launch_v<KIND, M, 4>(tc, a, nlaunch, st); break;
-    case 5: launch_v<KIND, M, 5>(tc, a, nlaunch, st); break;
-    default: launch_v<KIND, M, 6>(tc, a, nlaunch, st); break;
+    default: launch_v<KIND, M, 3>(tc, a, nlaunch, st); break;
     }
 }
 
