@@ -17,6 +17,8 @@ using namespace hetd;
 struct hetd_ctx {
     hetd_params p{};
     int nx = 0, ny = 0, nz = 0, nxh = 0;
+    int ldh = 0;    // row stride of half spectra: nxh rounded up to 16 (128-byte rows)
+    long nhp = 0;   // ldh*ny*nz, the allocated half-spectrum size
     long n = 0, nh = 0;
     double hx = 0, hy = 0, hz = 0, dxyz = 0, mAg = 0, renor = 0;
     bool otc = false;
@@ -134,6 +136,7 @@ static void tend(hetd_ctx* c, int k, double bytes) {
 // launches with their algorithmic HBM bytes (one read of each input, one
 // write of each output; half spectra have nxh*ny*nz points)
 template <typename T> static hipError_t run_x(hetd_ctx* c, XPassArgs<T> x) {
+    x.ldh = c->ldh;
     const double n = (double)c->n, nh = (double)c->nh, s = (double)sizeof(T);
     double b = 0;
     switch (x.mode) {
@@ -149,6 +152,7 @@ template <typename T> static hipError_t run_x(hetd_ctx* c, XPassArgs<T> x) {
 }
 
 template <typename T> static hipError_t run_y(hetd_ctx* c, YPassArgs<T> y, int nbuf) {
+    y.ldh = c->ldh;
     tbeg(c, HETD_K_YPASS);
     hipError_t e = reg_fft_supported(y.ax.n) ? launch_yreg(y, c->nz, nbuf, c->st) : launch_ypass(y, c->nz, nbuf, c->st);
     tend(c, HETD_K_YPASS, 4.0 * sizeof(T) * c->nh * nbuf);
@@ -156,6 +160,7 @@ template <typename T> static hipError_t run_y(hetd_ctx* c, YPassArgs<T> y, int n
 }
 
 template <typename T> static hipError_t run_z(hetd_ctx* c, ZPassArgs<T> z, int nbuf) {
+    z.ldh = c->ldh;
     const double s = (double)sizeof(T);
     double per = 4 * s;
     if (z.mode == ZMODE_CONV)
@@ -636,6 +641,8 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
     c->nx = p->nx; c->ny = p->ny; c->nz = p->nz; c->nxh = p->nx / 2 + 1;
     c->n = (long)c->nx * c->ny * c->nz;
     c->nh = (long)c->nxh * c->ny * c->nz;
+    c->ldh = (c->nxh + 15) / 16 * 16;
+    c->nhp = (long)c->ldh * c->ny * c->nz;
     c->hx = 2.0 * std::fabs(p->xmax) / p->nx;
     c->hy = 2.0 * std::fabs(p->ymax) / p->ny;
     c->hz = 2.0 * std::fabs(p->zmax) / p->nz;
@@ -692,7 +699,7 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
         }
     }
     // fields
-    const size_t nR = sizeof(double) * c->n, nC = sizeof(double2) * c->n, nH = sizeof(double2) * c->nh;
+    const size_t nR = sizeof(double) * c->n, nC = sizeof(double2) * c->n, nH = sizeof(double2) * c->nhp;
     HCHK(c, dalloc(c, &c->psi, nC));
     for (int i = 0; i < 3; ++i) HCHK(c, dalloc(c, &c->psiold[i], nC));
     HCHK(c, dalloc(c, &c->spare, nC));
@@ -707,16 +714,16 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
     HCHK(c, dalloc(c, &c->A, nH));
     HCHK(c, dalloc(c, &c->B, nH));
     HCHK(c, dalloc(c, &c->Cs, nH));
-    HCHK(c, dalloc(c, &c->fvlj, sizeof(double) * c->nh));
-    HCHK(c, dalloc(c, &c->wcgk, sizeof(double) * c->nh));
-    HCHK(c, dalloc(c, &c->kalfs, sizeof(double) * c->nh));
+    HCHK(c, dalloc(c, &c->fvlj, sizeof(double) * c->nhp));
+    HCHK(c, dalloc(c, &c->wcgk, sizeof(double) * c->nhp));
+    HCHK(c, dalloc(c, &c->kalfs, sizeof(double) * c->nhp));
     if (c->otc) {
         double** arr[] = {&c->dxden, &c->dyden, &c->dzden, &c->denalf, &c->falfs, &c->intx, &c->inty, &c->intz,
                           &c->ualphas, &c->s1, &c->s2, &c->s3, &c->s4, &c->s5, &c->s6};
         for (auto a : arr) HCHK(c, dalloc(c, a, nR));
     }
     if (c->mixed) {
-        const size_t fR = sizeof(float) * c->n, fH = sizeof(float2) * c->nh, fK = sizeof(float) * c->nh;
+        const size_t fR = sizeof(float) * c->n, fH = sizeof(float2) * c->nhp, fK = sizeof(float) * c->nhp;
         HCHK(c, dalloc(c, &c->den32, fR));
         HCHK(c, dalloc(c, &c->pot4_32, fR));
         HCHK(c, dalloc(c, &c->dencg32, fR));
@@ -738,10 +745,10 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
     HCHK(c, hipMemset(c->guard, 0, sizeof(int)));
     // form factors on the reference p-grid (fforma.f90, initcg.f90, DFT4He3d.f90:592-596)
     {
-        std::vector<double> fv(c->nh), wc(c->nh), ka(c->nh);
+        std::vector<double> fv(c->nhp, 0.0), wc(c->nhp, 0.0), ka(c->nhp, 0.0);
         const double hcg = p->lsolid ? p->h4 * 1.065 : p->h4;
         const double pi = 4.0 * std::atan(1.0);
-        const int nxh = c->nxh, ny = c->ny, nz = c->nz;
+        const int nxh = c->nxh, ldh = c->ldh, ny = c->ny, nz = c->nz;
         const double hx = c->hx, hy = c->hy, hz = c->hz;
         setup::parallel_for(nz, [&](int k) {
             const double pz = setup::pgrid(k, nz, hz);
@@ -750,20 +757,20 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
                 for (int i = 0; i < nxh; ++i) {
                     const double px = (1.0 / (c->nx * hx)) * i;  // hpx*(ipx-1), ipx <= nx/2+1
                     const double pm = std::sqrt(px * px + py * py + pz * pz);
-                    const size_t o = (size_t)i + (size_t)nxh * ((size_t)j + (size_t)ny * k);
+                    const size_t o = (size_t)i + (size_t)ldh * ((size_t)j + (size_t)ny * k);
                     fv[o] = setup::vlj_hat(pm, p->eps4, p->sigma4, p->h4);
                     wc[o] = setup::wcg(pm, hcg);
                     ka[o] = std::exp(-(pi * p->l_alpha * pm) * (pi * p->l_alpha * pm));
                 }
             }
         });
-        HCHK(c, hipMemcpy(c->fvlj, fv.data(), sizeof(double) * c->nh, hipMemcpyHostToDevice));
-        HCHK(c, hipMemcpy(c->wcgk, wc.data(), sizeof(double) * c->nh, hipMemcpyHostToDevice));
-        HCHK(c, hipMemcpy(c->kalfs, ka.data(), sizeof(double) * c->nh, hipMemcpyHostToDevice));
+        HCHK(c, hipMemcpy(c->fvlj, fv.data(), sizeof(double) * c->nhp, hipMemcpyHostToDevice));
+        HCHK(c, hipMemcpy(c->wcgk, wc.data(), sizeof(double) * c->nhp, hipMemcpyHostToDevice));
+        HCHK(c, hipMemcpy(c->kalfs, ka.data(), sizeof(double) * c->nhp, hipMemcpyHostToDevice));
         if (c->mixed) {
-            HCHK(c, launch_to_float(c->fvlj, c->fvlj32, c->nh, c->st));
-            HCHK(c, launch_to_float(c->wcgk, c->wcgk32, c->nh, c->st));
-            HCHK(c, launch_to_float(c->kalfs, c->kalfs32, c->nh, c->st));
+            HCHK(c, launch_to_float(c->fvlj, c->fvlj32, c->nhp, c->st));
+            HCHK(c, launch_to_float(c->wcgk, c->wcgk32, c->nhp, c->st));
+            HCHK(c, launch_to_float(c->kalfs, c->kalfs32, c->nhp, c->st));
             HCHK(c, hipStreamSynchronize(c->st));
         }
     }
@@ -988,9 +995,10 @@ static const double* widen(hetd_ctx* c, const float* f, long n, double* dst) {
 
 int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
     if (!c || !host) return HETD_ERR_ARG;
-    const size_t nR = sizeof(double) * c->n, nC = sizeof(double2) * c->n, nHr = sizeof(double) * c->nh;
+    const size_t nR = sizeof(double) * c->n, nC = sizeof(double2) * c->n;
     const void* src = nullptr;
     size_t bytes = nR;
+    size_t hrow = 0;  // element size of a half-spectrum field (rows padded to ldh)
     int rc = 0;
     const bool mx = c->mixed;
     if ((rc = ensure_den64(c))) return rc;
@@ -1005,8 +1013,8 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
         break;
     case HETD_F_PC: src = c->pc; bytes = nC; break;
     case HETD_F_Q: src = c->q; bytes = nC; break;
-    case HETD_F_FVLJ: src = c->fvlj; bytes = nHr; break;
-    case HETD_F_WCGK: src = c->wcgk; bytes = nHr; break;
+    case HETD_F_FVLJ: src = c->fvlj; hrow = sizeof(double); break;
+    case HETD_F_WCGK: src = c->wcgk; hrow = sizeof(double); break;
     case HETD_F_CCOR:
     case HETD_F_FDEN: {
         // recompute the last poten in its full form (den is unchanged since)
@@ -1019,9 +1027,9 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
         if (which == HETD_F_CCOR) {
             src = mx ? widen(c, c->tmp32, c->n, c->tmp) : (const void*)c->tmp;
         } else {
-            src = mx ? widen(c, reinterpret_cast<const float*>(c->C32), 2 * c->nh, reinterpret_cast<double*>(c->A))
+            src = mx ? widen(c, reinterpret_cast<const float*>(c->C32), 2 * c->nhp, reinterpret_cast<double*>(c->A))
                      : (const void*)c->Cs;
-            bytes = sizeof(double2) * c->nh;
+            hrow = sizeof(double2);
         }
         break;
     }
@@ -1055,7 +1063,11 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
         return fail(c, HETD_ERR_ARG, "unknown field %d", which);
     }
     if (!src) return fail(c, HETD_ERR_HIP, "conversion launch failed");
-    HCHK(c, hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, c->st));
+    if (hrow)  // half spectrum: drop the row padding
+        HCHK(c, hipMemcpy2DAsync(host, hrow * c->nxh, src, hrow * c->ldh, hrow * c->nxh, (size_t)c->ny * c->nz,
+                                 hipMemcpyDeviceToHost, c->st));
+    else
+        HCHK(c, hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, c->st));
     HCHK(c, hipStreamSynchronize(c->st));
     return HETD_OK;
 }
@@ -1065,14 +1077,16 @@ int hetd_fft_r2c(hetd_ctx* c, const double* in, double* out) {
     HCHK(c, hipMemcpyAsync(c->tmp, in, sizeof(double) * c->n, hipMemcpyHostToDevice, c->st));
     int rc = r2c_dev(c, c->tmp, c->A);
     if (rc) return rc;
-    HCHK(c, hipMemcpyAsync(out, c->A, sizeof(double2) * c->nh, hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipMemcpy2DAsync(out, sizeof(double2) * c->nxh, c->A, sizeof(double2) * c->ldh, sizeof(double2) * c->nxh,
+                             (size_t)c->ny * c->nz, hipMemcpyDeviceToHost, c->st));
     HCHK(c, hipStreamSynchronize(c->st));
     return HETD_OK;
 }
 
 int hetd_fft_c2r(hetd_ctx* c, const double* in, double* out) {
     if (!c || !in || !out) return HETD_ERR_ARG;
-    HCHK(c, hipMemcpyAsync(c->A, in, sizeof(double2) * c->nh, hipMemcpyHostToDevice, c->st));
+    HCHK(c, hipMemcpy2DAsync(c->A, sizeof(double2) * c->ldh, in, sizeof(double2) * c->nxh, sizeof(double2) * c->nxh,
+                             (size_t)c->ny * c->nz, hipMemcpyHostToDevice, c->st));
     int rc = c2r_dev(c, c->A, c->tmp, c->renor);
     if (rc) return rc;
     HCHK(c, hipMemcpyAsync(out, c->tmp, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->st));

@@ -48,8 +48,8 @@ __global__ void __launch_bounds__(256) k_xpass(XPassArgs<T> a) {
             const int pp = idx / nx, k = idx % nx;
             const long la = 2 * (pair0 + pp), lb = la + 1;
             const int kk = k < nxh ? k : nx - k;
-            C A = a.cin[la * nxh + kk];
-            C B = lb < nlines ? a.cin[lb * nxh + kk] : cmk<C>((T)0, (T)0);
+            C A = a.cin[la * a.ldh + kk];
+            C B = lb < nlines ? a.cin[lb * a.ldh + kk] : cmk<C>((T)0, (T)0);
             if (kk == 0 || (2 * kk == nx)) { A.y = (T)0; B.y = (T)0; }
             if (k != kk) { A.y = -A.y; B.y = -B.y; }
             lds[pp * nx + k] = cmk<C>(A.x - B.y, A.y + B.x);  // A + iB
@@ -121,8 +121,8 @@ __global__ void __launch_bounds__(256) k_xpass(XPassArgs<T> a) {
         const T h = (T)0.5;
         C A = cmk<C>(h * (zk.x + zn.x), h * (zk.y - zn.y));
         C B = cmk<C>(h * (zk.y + zn.y), -h * (zk.x - zn.x));
-        a.cout[la * nxh + k] = A;
-        if (lb < nlines) a.cout[lb * nxh + k] = B;
+        a.cout[la * a.ldh + k] = A;
+        if (lb < nlines) a.cout[lb * a.ldh + k] = B;
     }
 }
 
@@ -133,20 +133,20 @@ __global__ void __launch_bounds__(256) k_ypass(YPassArgs<T> a) {
     using C = cplx<T>;
     extern __shared__ unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
-    const int ny = a.ax.n, TX = a.tx, nxh = a.nxh;
+    const int ny = a.ax.n, TX = a.tx, nxh = a.nxh, ldh = a.ldh;
     const int kx0 = blockIdx.x * TX, z = blockIdx.y;
     C* buf = a.buf[blockIdx.z];
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const long base = (long)nxh * ny * z;
+    const long base = (long)ldh * ny * z;
     for (int idx = tid; idx < ny * TX; idx += nthr) {
         const int j = idx / TX, t = idx % TX, kx = kx0 + t;
-        lds[idx] = kx < nxh ? buf[base + (long)nxh * j + kx] : cmk<C>((T)0, (T)0);
+        lds[idx] = kx < nxh ? buf[base + (long)ldh * j + kx] : cmk<C>((T)0, (T)0);
     }
     __syncthreads();
     fft_lds<T, PB, true>(lds, a.ax, TX, TX, 1, a.inv != 0, tid, nthr);
     for (int idx = tid; idx < ny * TX; idx += nthr) {
         const int j = idx / TX, t = idx % TX, kx = kx0 + t;
-        if (kx < nxh) buf[base + (long)nxh * j + kx] = lds[idx];
+        if (kx < nxh) buf[base + (long)ldh * j + kx] = lds[idx];
     }
 }
 
@@ -161,8 +161,8 @@ __global__ void __launch_bounds__(256) k_zpass(ZPassArgs<T> a) {
     const int kx0 = blockIdx.x * TX, ky = blockIdx.y;
     C* l1 = l0 + nz * TX;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const long zs = (long)nxh * ny;
-    const long base = (long)nxh * ky + kx0;
+    const long zs = (long)a.ldh * ny;
+    const long base = (long)a.ldh * ky + kx0;
     const C* in = a.in[blockIdx.z];
     for (int idx = tid; idx < nz * TX; idx += nthr) {
         const int j = idx / TX, t = idx % TX;

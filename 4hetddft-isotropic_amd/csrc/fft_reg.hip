@@ -147,8 +147,8 @@ __global__ void __launch_bounds__(R* TL) k_xreg(XPassArgs<T> a) {
         for (int j = 0; j < R; ++j) {
             const int k = c + R * j;
             const int kk = k < NH ? k : N - k;
-            C A = va ? a.cin[la * NH + kk] : cmk<C>((T)0, (T)0);
-            C B = vb ? a.cin[lb * NH + kk] : cmk<C>((T)0, (T)0);
+            C A = va ? a.cin[la * a.ldh + kk] : cmk<C>((T)0, (T)0);
+            C B = vb ? a.cin[lb * a.ldh + kk] : cmk<C>((T)0, (T)0);
             if (kk == 0 || 2 * kk == N) { A.y = (T)0; B.y = (T)0; }
             if (k != kk) { A.y = -A.y; B.y = -B.y; }
             v[j] = cmk<C>(A.x - B.y, A.y + B.x);
@@ -220,8 +220,8 @@ __global__ void __launch_bounds__(R* TL) k_xreg(XPassArgs<T> a) {
         if (k < NH) {
             const int km = (N - k) & (N - 1);
             const C zk = M[k + k / R], zn = M[km + km / R];
-            if (va) a.cout[la * NH + k] = cmk<C>(h * (zk.x + zn.x), h * (zk.y - zn.y));
-            if (vb) a.cout[lb * NH + k] = cmk<C>(h * (zk.y + zn.y), -h * (zk.x - zn.x));
+            if (va) a.cout[la * a.ldh + k] = cmk<C>(h * (zk.x + zn.x), h * (zk.y - zn.y));
+            if (vb) a.cout[lb * a.ldh + k] = cmk<C>(h * (zk.y + zn.y), -h * (zk.x - zn.x));
         }
     }
 }
@@ -237,21 +237,21 @@ __global__ void __launch_bounds__(R* TX) k_yreg(YPassArgs<T> a) {
     __shared__ C tw[N];
     const int tid = threadIdx.x, l = tid % TX, c = tid / TX;
     for (int e = tid; e < N; e += R * TX) tw[e] = twiddle<T>(a.ax, e);
-    const int nxh = a.nxh, kx = blockIdx.x * TX + l, z = blockIdx.y;
+    const int nxh = a.nxh, ldh = a.ldh, kx = blockIdx.x * TX + l, z = blockIdx.y;
     C* buf = a.buf[blockIdx.z];
-    const long base = (long)nxh * N * z + kx;
+    const long base = (long)ldh * N * z + kx;
     const bool ok = kx < nxh;
     const bool inv = a.inv != 0;
     C v[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = ok ? buf[base + (long)nxh * (c + R * j)] : cmk<C>((T)0, (T)0);
+    for (int j = 0; j < R; ++j) v[j] = ok ? buf[base + (long)ldh * (c + R * j)] : cmk<C>((T)0, (T)0);
     __syncthreads();
     col_phase<C, R>(v, c, lds + l, R * TX, TX, tw, inv);
     __syncthreads();
     row_phase<C, R>(v, c, lds + l, R * TX, TX, inv);
     if (ok)
 #pragma unroll
-        for (int k = 0; k < R; ++k) buf[base + (long)nxh * (c + R * k)] = v[k];
+        for (int k = 0; k < R; ++k) buf[base + (long)ldh * (c + R * k)] = v[k];
 }
 
 // ------------------------------------------------------------------ z pass
@@ -267,8 +267,8 @@ __global__ void __launch_bounds__(R* TX) k_zreg(ZPassArgs<T> a) {
     const int tid = threadIdx.x, l = tid % TX, c = tid / TX;
     for (int e = tid; e < N; e += R * TX) tw[e] = twiddle<T>(a.ax, e);
     const int nxh = a.nxh, kx = blockIdx.x * TX + l, ky = blockIdx.y;
-    const long zs = (long)nxh * a.ny;
-    const long base = (long)nxh * ky + kx;
+    const long zs = (long)a.ldh * a.ny;
+    const long base = (long)a.ldh * ky + kx;
     const bool ok = kx < nxh;
     const C* in = a.in[blockIdx.z];
     C* lt = lds + l;

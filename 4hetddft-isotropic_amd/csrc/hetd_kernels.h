@@ -13,6 +13,7 @@ enum ZMode { ZMODE_FWD = 0, ZMODE_INV = 1, ZMODE_CONV = 2 };
 template <typename T> struct XPassArgs {
     FftAxis ax;
     long nlines;  // ny*nz
+    int ldh;      // row stride of the half spectra (>= nx/2+1, padded for aligned rows)
     int ppb;      // line pairs per block (set by the launcher)
     int mode;
     T scale;      // applied after an inverse transform (renor)
@@ -30,13 +31,13 @@ template <typename T> struct XPassArgs {
 
 template <typename T> struct YPassArgs {
     FftAxis ax;
-    int nxh, tx, inv;
+    int nxh, ldh, tx, inv;  // kx count, row stride
     cplx<T>* buf[4];
 };
 
 template <typename T> struct ZPassArgs {
     FftAxis ax;
-    int nxh, ny, tx, mode, nk;
+    int nxh, ldh, ny, tx, mode, nk;  // kx count, row stride
     const cplx<T>* in[4];
     cplx<T>* out[4][3];
     const T* kern[3];

@@ -61,6 +61,23 @@ class HipEvents:
         assert self.L.hipEventElapsedTime(C.byref(ms), a, b) == 0
         return ms.value
 
+    def copy_calibration(self, mib, reps=3):
+        """Device-to-device copies of `mib` MiB: the known-size reference
+        for scaling FETCH_SIZE / WRITE_SIZE counters to bytes."""
+        L = self.L
+        L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        L.hipFree.argtypes = [C.c_void_p]
+        nb = mib << 20
+        a, b = C.c_void_p(), C.c_void_p()
+        assert L.hipMalloc(C.byref(a), nb) == 0 and L.hipMalloc(C.byref(b), nb) == 0
+        for _ in range(reps):
+            assert L.hipMemcpy(b, a, nb, 3) == 0  # hipMemcpyDeviceToDevice
+        assert L.hipDeviceSynchronize() == 0
+        L.hipFree(a)
+        L.hipFree(b)
+        return nb
+
 
 def cpu_baseline(n, nthreads, steps=2):
     """Reference CPU path on this host's cores, a bounded sample: `steps`
@@ -204,6 +221,8 @@ def main():
     ap.add_argument("--kernel", default="update", choices=["update", "xpass", "ypass", "zpass", "force"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run device copies of MIB MiB (the byte reference for PMC traffic counters)")
     ap.add_argument("--check-steps", type=int, default=400,
                     help="untimed mixed-vs-FP64 accuracy run length (0: skip)")
     args = ap.parse_args()
@@ -220,6 +239,8 @@ def main():
     kclass = {"update": capi.K_UPDATE, "xpass": capi.K_XPASS, "ypass": capi.K_YPASS,
               "zpass": capi.K_ZPASS, "force": capi.K_FORCE}[args.kernel]
     ev = HipEvents()
+    if args.calibrate_copy:
+        print("copy calibration: %d bytes per copy" % ev.copy_calibration(args.calibrate_copy), file=sys.stderr)
     res = {}
     if args.precision in ("both", "fp64"):
         res["fp64"] = run_precision(args, capi.HETD_FP64, psi, kclass, ev)
