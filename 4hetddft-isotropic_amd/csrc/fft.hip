@@ -86,22 +86,14 @@ __global__ void __launch_bounds__(256) k_xpass(XPassArgs<T> a) {
                 {
                     const T g = a.rin2[ia];
                     T u = (a.rin3 ? a.rin3[ia] : (T)0) + g * g * (a.a0 + a.a1 * g) + ca;
-                    if (a.lsolid) {
-                        const T d = a.rin[ia];
-                        const T t = tanh(a.beta * (d - a.den_m));
-                        u = u + a.csol * ((T)1 + t + a.beta * d * ((T)1 - t * t));
-                    }
+                    if (a.lsolid) u = u + solid_penalty<T>(a.rin[ia], a.csol, a.beta, a.den_m);
                     if (a.rout) a.rout[ia] = u;
                     if (a.rout2) a.rout2[ia] = ca;
                 }
                 if (hb) {
                     const T g = a.rin2[ib];
                     T u = (a.rin3 ? a.rin3[ib] : (T)0) + g * g * (a.a0 + a.a1 * g) + cb;
-                    if (a.lsolid) {
-                        const T d = a.rin[ib];
-                        const T t = tanh(a.beta * (d - a.den_m));
-                        u = u + a.csol * ((T)1 + t + a.beta * d * ((T)1 - t * t));
-                    }
+                    if (a.lsolid) u = u + solid_penalty<T>(a.rin[ib], a.csol, a.beta, a.den_m);
                     if (a.rout) a.rout[ib] = u;
                     if (a.rout2) a.rout2[ib] = cb;
                 }

@@ -20,6 +20,20 @@ template <> struct cplx_t<float> { using type = float2; };
 template <typename T> using cplx = typename cplx_t<T>::type;
 
 template <typename C> __device__ __forceinline__ C cmk(decltype(C::x) a, decltype(C::x) b) { C r; r.x = a; r.y = b; return r; }
+// Solid penalty C_s (1 + t + beta d (1 - t^2)), t = tanh(beta (d - den_m))
+// (poten.f90:80-83), through s = exp(-2|x|):
+//   1 + t = 2/(1+s) (x >= 0) or 2s/(1+s) (x < 0),   1 - t^2 = 4s/(1+s)^2,
+// which has no cancellation where t is near -1 (all of a He density, where
+// 1 + t ~ 1e-13) and costs one exp and one division instead of a tanh.
+template <typename T> __device__ __forceinline__ T solid_penalty(T d, T csol, T beta, T den_m) {
+    const T x = beta * (d - den_m);
+    const T sx = exp((T)-2 * fabs(x));
+    const T r = (T)1 / ((T)1 + sx);
+    const T onept = x >= (T)0 ? (T)2 * r : (T)2 * sx * r;
+    const T omt2 = (T)4 * sx * r * r;
+    return csol * (onept + beta * d * omt2);
+}
+
 // |z|^2 and |z|: x*x + y*y with one rounding fewer than hypot(x,y)^2 (the
 // reference's abs(z)**2); the two differ by at most an ulp or two
 __device__ __forceinline__ double abs2(double2 v) { return fma(v.x, v.x, v.y * v.y); }

@@ -26,6 +26,14 @@ template <> struct vz<double> {
     static __device__ __forceinline__ double zero() { return 0.0; }
     static __device__ __forceinline__ double add(double a, double b) { return a + b; }
     static __device__ __forceinline__ double sub(double a, double b) { return a - b; }
+    // v, -v or +0 for mode 1, -1, 0, by bit operations on v (so that a load
+    // feeding it is never moved under a branch)
+    static __device__ __forceinline__ double mode(double v, int m) {
+        unsigned long long b = (unsigned long long)__double_as_longlong(v);
+        b &= 0ull - (unsigned long long)(m != 0);
+        b ^= (unsigned long long)(m < 0) << 63;
+        return __longlong_as_double((long long)b);
+    }
     static __device__ __forceinline__ double scale(double v, double s) { return v * s; }
     static __device__ __forceinline__ void fma_(double& s, double c, double v) { s = fma(c, v, s); }
 };
@@ -33,37 +41,15 @@ template <> struct vz<double2> {
     static __device__ __forceinline__ double2 zero() { return make_double2(0.0, 0.0); }
     static __device__ __forceinline__ double2 add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
     static __device__ __forceinline__ double2 sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+    static __device__ __forceinline__ double2 mode(double2 v, int m) {
+        return make_double2(vz<double>::mode(v.x, m), vz<double>::mode(v.y, m));
+    }
     static __device__ __forceinline__ double2 scale(double2 v, double s) { return make_double2(v.x * s, v.y * s); }
     static __device__ __forceinline__ void fma_(double2& s, double c, double2 v) {
         s.x = fma(c, v.x, s.x);
         s.y = fma(c, v.y, s.y);
     }
 };
-
-// value of f at (ix, iy, iz) where at most one coordinate may be outside the
-// box: zero (icon 13) or wrapped (icon 8; icon 18 flips the sign)
-template <typename V>
-__device__ __forceinline__ V load_bc(const V* __restrict__ f, int nx, int ny, int nz, int ix, int iy, int iz,
-                                     int icon) {
-    double sg = 1.0;
-    if (ix < 0 || ix >= nx) {
-        if (icon == 13) return vz<V>::zero();
-        ix = ix < 0 ? ix + nx : ix - nx;
-        if (icon == 18) sg = -sg;
-    }
-    if (iy < 0 || iy >= ny) {
-        if (icon == 13) return vz<V>::zero();
-        iy = iy < 0 ? iy + ny : iy - ny;
-        if (icon == 18) sg = -sg;
-    }
-    if (iz < 0 || iz >= nz) {
-        if (icon == 13) return vz<V>::zero();
-        iz = iz < 0 ? iz + nz : iz - nz;
-        if (icon == 18) sg = -sg;
-    }
-    const V v = f[(long)ix + (long)nx * ((long)iy + (long)ny * iz)];
-    return sg == 1.0 ? v : vz<V>::scale(v, sg);
-}
 
 enum TKind { TK_UPDATE = 0, TK_FORCE = 1, TK_ENERGY = 2 };
 
@@ -88,21 +74,6 @@ struct TiledCommon {
     double w1h[kMaxNpd / 2 + 1], w2h[kMaxNpd / 2 + 1];
     double s1[3], s2[3];
 };
-
-// value of f at grid column (gx, gy), plane iz (in range), for an LDS cell:
-// the boundary rule when one of gx, gy is outside the box, zero for cells no
-// valid point reads (corners, beyond one halo width, axes shorter than the
-// stencil, where the reference's derivative is zero)
-template <typename V, int M>
-__device__ __forceinline__ V cell_value(const V* __restrict__ f, int nx, int ny, int nz, int gx, int gy, int iz,
-                                        int icon) {
-    constexpr int W = 2 * M + 1;
-    const bool ox = gx < 0 || gx >= nx, oy = gy < 0 || gy >= ny;
-    if (ox && oy) return vz<V>::zero();
-    if (ox && (nx < W || gx < -M || gx >= nx + M)) return vz<V>::zero();
-    if (oy && (ny < W || gy < -M || gy >= ny + M)) return vz<V>::zero();
-    return load_bc<V>(f, nx, ny, nz, gx, gy, iz, icon);
-}
 
 __device__ __forceinline__ double dens(double2 v) { return abs2(v); }
 
@@ -163,102 +134,153 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
 #pragma unroll
     for (int c = 0; c < NC; ++c) acc[c] = 0.0;
 
+    // Every boundary decision is taken once here, not per plane: a cell of
+    // the LDS plane is (offset within a z plane, mode) with mode 0 = zero,
+    // 1 = value, -1 = negated value (icon 18 across one face); the z queue of
+    // a point needs a rule only for planes beyond the box (uniform per block).
+    const long pl = (long)nx * ny;
+    const bool ragged = ix0 + TTX > nx || iy0 + TTY > ny;
     int iy[TPY];
     bool valid[TPY];
+    long c0[TPY];     // in-plane offset of the point (valid) or of its cell
+    int cm[TPY];      // mode of that cell when the point is not valid
     V q[TPY][W];
     double dq[FRC ? TPY : 1][W];  // |psi|^2 of the z queue
+    auto cell_rule = [&](int gx, int gy, long& off, int& mode) {
+        // column (gx, gy) of an LDS cell: inside the box, or across one face
+        // within one halo width on an axis at least as long as the stencil
+        // (zero outside for icon 13, wrapped for 8 and 18, negated for 18);
+        // anything else is zero, as no valid point reads it (corners) or the
+        // reference's derivative along that axis is zero
+        const bool ox = gx < 0 || gx >= nx, oy = gy < 0 || gy >= ny;
+        mode = 1;
+        if ((ox && oy) || (ox && (nx < W || gx < -M || gx >= nx + M)) ||
+            (oy && (ny < W || gy < -M || gy >= ny + M)) || ((ox || oy) && icon == 13)) {
+            mode = 0;
+            off = 0;
+            return;
+        }
+        if (ox) gx = gx < 0 ? gx + nx : gx - nx;
+        if (oy) gy = gy < 0 ? gy + ny : gy - ny;
+        if ((ox || oy) && icon == 18) mode = -1;
+        off = (long)gx + (long)nx * gy;
+    };
+    // Loads are unconditional (a zero-mode cell reads a valid address and
+    // discards it): a load under a branch makes the compiler wait for it at
+    // the end of the branch, which would serialise the prefetches.
+    auto fetch = [&](long off, int mode) -> V { return vz<V>::mode(f[off], mode); };
+    // the point's own column at plane zz (any zz): zero, or one z wrap.
+    // zsrc gives the address and sets the mode; the mode is applied where the
+    // value is used, so that a prefetch is not waited for where it is issued.
+    auto zsrc = [&](int p, int zz, int& m) -> long {
+        int zw = zz, zm = 1;
+        if (zz < 0 || zz >= nz) {  // uniform over the workgroup
+            if (icon == 13 || !zok) {
+                zm = 0;
+                zw = 0;
+            } else {
+                zw = zz < 0 ? zz + nz : zz - nz;
+                if (icon == 18) zm = -1;
+            }
+        }
+        m = valid[p] ? zm : 0;
+        return c0[p] + pl * zw;
+    };
+    auto zcol = [&](int p, int zz) -> V {
+        int m;
+        const long o = zsrc(p, zz, m);
+        return fetch(o, m);
+    };
 #pragma unroll
     for (int p = 0; p < TPY; ++p) {
         iy[p] = iy0 + ty + RS * p;
         valid[p] = ix < nx && iy[p] < ny;
+        cm[p] = 1;
+        if (valid[p]) c0[p] = (long)ix + (long)nx * iy[p];
+        else cell_rule(ix, iy[p], c0[p], cm[p]);
 #pragma unroll
         for (int k = 0; k < W - 1; ++k) {
-            const int zz = iz0 - M + k;
-            q[p][k] = (valid[p] && (zok || (zz >= 0 && zz < nz))) ? load_bc<V>(f, nx, ny, nz, ix, iy[p], zz, icon)
-                                                                   : vz<V>::zero();
+            q[p][k] = zcol(p, iz0 - M + k);
             if constexpr (FRC) dq[p][k] = dens(q[p][k]);
         }
     }
 
     // halo cells of this thread (slot k: cell threadIdx.x + k*TNT of the
-    // ring); x halo = M columns on each side of the tile rows, y halo = M
-    // rows above and below the tile columns (corners are never read)
-    constexpr int NHX = 2 * M * TTY, NH = NHX + 2 * M * TTX, KH = (NH + TNT - 1) / TNT;
+    // ring); y halo = M rows above and below the tile columns first, then the
+    // x halo = M columns on each side of the tile rows (corners are never read)
+    constexpr int NHY = 2 * M * TTX, NH = NHY + 2 * M * TTY, KH = (NH + TNT - 1) / TNT;
     int hl[KH];  // LDS index of slot k, -1 if unused
-    int hgx[KH], hgy[KH];
+    long hoff[KH];
+    int hm[KH];
 #pragma unroll
     for (int k = 0; k < KH; ++k) {
         const int h = threadIdx.x + k * TNT;
         hl[k] = -1;
+        hoff[k] = 0;
+        hm[k] = 0;
         if (h < NH) {
             int lx, ly;
-            if (h < NHX) {
-                const int r = h / (2 * M), cc = h % (2 * M);
-                ly = r + M;
-                lx = cc < M ? cc : TTX + cc;
-            } else {
-                const int hh = h - NHX;
-                const int r = hh / TTX, cc = hh % TTX;
+            if (h < NHY) {
+                const int r = h / TTX, cc = h % TTX;
                 lx = cc + M;
                 ly = r < M ? r : TTY + r;
+            } else {
+                const int hh = h - NHY;
+                const int r = hh / (2 * M), cc = hh % (2 * M);
+                ly = r + M;
+                lx = cc < M ? cc : TTX + cc;
             }
             hl[k] = ly * LX + lx;
-            hgx[k] = ix0 + lx - M;
-            hgy[k] = iy0 + ly - M;
+            cell_rule(ix0 + lx - M, iy0 + ly - M, hoff[k], hm[k]);
         }
     }
     // software pipeline: the halo of plane iz and the queue fronts are
     // loaded one iteration ahead, while plane iz-1 is being computed
-    V hreg[KH], qn[TPY];
+    V hraw[KH], qraw[TPY];  // prefetched raw values; modes hm[] and qm[]
+    int qm[TPY];
 #pragma unroll
-    for (int k = 0; k < KH; ++k)
-        hreg[k] = PF && hl[k] >= 0 && iz0 < izend ? cell_value<V, M>(f, nx, ny, nz, hgx[k], hgy[k], iz0, icon)
-                                                  : vz<V>::zero();
+    for (int k = 0; k < KH; ++k) hraw[k] = PF && iz0 < izend ? f[hoff[k] + pl * iz0] : vz<V>::zero();
 #pragma unroll
     for (int p = 0; p < TPY; ++p) {
-        const int zz = iz0 + M;
-        qn[p] = (PF && valid[p] && (zok || zz < nz)) ? load_bc<V>(f, nx, ny, nz, ix, iy[p], zz, icon)
-                                                     : vz<V>::zero();
+        qm[p] = 0;
+        qraw[p] = vz<V>::zero();
+        if (PF) qraw[p] = f[zsrc(p, iz0 + M, qm[p])];
     }
 
     for (int iz = iz0; iz < izend; ++iz) {
 #pragma unroll
         for (int p = 0; p < TPY; ++p) {
-            if (!PF) {
-                const int zz = iz + M;
-                qn[p] = (valid[p] && (zok || zz < nz)) ? load_bc<V>(f, nx, ny, nz, ix, iy[p], zz, icon)
-                                                        : vz<V>::zero();
-            }
-            q[p][W - 1] = qn[p];
-            if constexpr (FRC) dq[p][W - 1] = dens(qn[p]);
+            const V qn = PF ? vz<V>::mode(qraw[p], qm[p]) : zcol(p, iz + M);
+            q[p][W - 1] = qn;
+            if constexpr (FRC) dq[p][W - 1] = dens(qn);
         }
         __syncthreads();
         // plane iz into LDS: interior from the queues, halo from the prefetch
 #pragma unroll
         for (int p = 0; p < TPY; ++p) {
             const int li = (ty + RS * p + M) * LX + tx + M;
-            const V v = valid[p] ? q[p][M] : cell_value<V, M>(f, nx, ny, nz, ix, iy[p], iz, icon);
-            lds[li] = v;
-            if constexpr (FRC) ldsd[li] = valid[p] ? dq[p][M] : dens(v);
+            if (ragged) {  // block-uniform: some of its points lie beyond the box
+                const V v = valid[p] ? q[p][M] : fetch(c0[p] + pl * iz, cm[p]);
+                lds[li] = v;
+                if constexpr (FRC) ldsd[li] = valid[p] ? dq[p][M] : dens(v);
+            } else {
+                lds[li] = q[p][M];
+                if constexpr (FRC) ldsd[li] = dq[p][M];
+            }
         }
 #pragma unroll
         for (int k = 0; k < KH; ++k)
             if (hl[k] >= 0) {
-                const V v = PF ? hreg[k] : cell_value<V, M>(f, nx, ny, nz, hgx[k], hgy[k], iz, icon);
+                const V v = PF ? vz<V>::mode(hraw[k], hm[k]) : fetch(hoff[k] + pl * iz, hm[k]);
                 lds[hl[k]] = v;
                 if constexpr (FRC) ldsd[hl[k]] = dens(v);
             }
         __syncthreads();
         if (PF && iz + 1 < izend) {
 #pragma unroll
-            for (int k = 0; k < KH; ++k)
-                if (hl[k] >= 0) hreg[k] = cell_value<V, M>(f, nx, ny, nz, hgx[k], hgy[k], iz + 1, icon);
+            for (int k = 0; k < KH; ++k) hraw[k] = f[hoff[k] + pl * (iz + 1)];
 #pragma unroll
-            for (int p = 0; p < TPY; ++p) {
-                const int zz = iz + 1 + M;
-                qn[p] = (valid[p] && (zok || zz < nz)) ? load_bc<V>(f, nx, ny, nz, ix, iy[p], zz, icon)
-                                                        : vz<V>::zero();
-            }
+            for (int p = 0; p < TPY; ++p) qraw[p] = f[zsrc(p, iz + 1 + M, qm[p])];
         }
 #pragma unroll
         for (int p = 0; p < TPY; ++p) {
