@@ -67,6 +67,12 @@ struct hetd_ctx {
     int ioldp[3] = {1, 2, 3}, ioldh[2] = {1, 2}, ioldr[3] = {1, 2, 3}, ioldv[3] = {1, 2, 3}, iolda[2] = {1, 2};
     long iter_since_set = 0;
     bool have_state = false;
+    // predictor-corrector steps replayed from a HIP graph (see step_graph)
+    bool use_graphs = true;
+    hipGraphExec_t gexec = nullptr;
+    int gper = 0;        // steps per replay (the period of the buffer rotation)
+    double gdt = 0.0;
+    std::vector<long> gsig;  // host state the replay starts from (and ends in)
     // kernel-class timing (hetd_timing)
     int tclass = 0;
     std::vector<hipEvent_t> tev;
@@ -572,6 +578,134 @@ static int check_guard(hetd_ctx* c) {
     return 0;
 }
 
+// ------------------------------------------------------------ step graphs
+//
+// A steady-state iteration (steppc + the driver's potential update) is the
+// same ~25 launches every step, on buffers that rotate with a short period
+// (psi/psiold/spare pointers, the ring indices, the deferred-force flag).
+// The launches of one full period are captured once into a HIP graph and
+// replayed: one launch per period instead of ~25 per step, which is what
+// small grids (and an ensemble of them) are bound by.  The graph holds no
+// memset or host synchronisation; replays are bitwise identical to the
+// launches they record.
+
+// every host-side value a step reads or rotates
+static std::vector<long> step_sig(const hetd_ctx* c) {
+    std::vector<long> v;
+    v.push_back((long)(intptr_t)c->psi);
+    for (int i = 0; i < 3; ++i) v.push_back((long)(intptr_t)c->psiold[i]);
+    v.push_back((long)(intptr_t)c->spare);
+    for (int i = 0; i < 2; ++i) v.push_back((long)(intptr_t)c->hpsiold[i]);
+    for (int i = 0; i < 3; ++i) v.push_back(c->ioldp[i]);
+    for (int i = 0; i < 2; ++i) v.push_back(c->ioldh[i]);
+    for (int i = 0; i < 3; ++i) v.push_back(c->ioldr[i]);
+    for (int i = 0; i < 3; ++i) v.push_back(c->ioldv[i]);
+    for (int i = 0; i < 2; ++i) v.push_back(c->iolda[i]);
+    v.push_back(c->force_pending);
+    v.push_back(c->delj4_state);
+    v.push_back(c->den64_ok);
+    return v;
+}
+
+static void restore_sig(hetd_ctx* c, const std::vector<long>& v) {
+    size_t k = 0;
+    c->psi = (double2*)(intptr_t)v[k++];
+    for (int i = 0; i < 3; ++i) c->psiold[i] = (double2*)(intptr_t)v[k++];
+    c->spare = (double2*)(intptr_t)v[k++];
+    for (int i = 0; i < 2; ++i) c->hpsiold[i] = (double2*)(intptr_t)v[k++];
+    for (int i = 0; i < 3; ++i) c->ioldp[i] = (int)v[k++];
+    for (int i = 0; i < 2; ++i) c->ioldh[i] = (int)v[k++];
+    for (int i = 0; i < 3; ++i) c->ioldr[i] = (int)v[k++];
+    for (int i = 0; i < 3; ++i) c->ioldv[i] = (int)v[k++];
+    for (int i = 0; i < 2; ++i) c->iolda[i] = (int)v[k++];
+    c->force_pending = v[k++] != 0;
+    c->delj4_state = (int)v[k++];
+    c->den64_ok = v[k++] != 0;
+}
+
+// one steady-state iteration of the driver loop (DFT4He3d.f90:844-852)
+static int pc_iteration(hetd_ctx* c, double dt) {
+    int rc = steppc_dev(c, dt, true);
+    if (rc) return rc;
+    return potential_update(c, true);
+}
+
+static void drop_graph(hetd_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gper = 0;
+    c->gsig.clear();
+}
+
+// Capture one period of PC iterations from the current host state (nothing
+// executes; the host state is left as it was).  Returns 0 with c->gexec set,
+// or 0 with c->gexec null when no period of at most 12 steps was found.
+static int build_step_graph(hetd_ctx* c, double dt) {
+    drop_graph(c);
+    const std::vector<long> s0 = step_sig(c);
+    HCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    int per = 0, rc = 0;
+    for (int k = 1; k <= 12 && !rc; ++k) {
+        rc = pc_iteration(c, dt);
+        if (!rc && step_sig(c) == s0) { per = k; break; }
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->st, &g);
+    restore_sig(c, s0);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    HCHK(c, e);
+    if (per > 0) {
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HCHK(c, ei);
+        c->gexec = x;
+        c->gper = per;
+        c->gdt = dt;
+        c->gsig = s0;
+    } else {
+        (void)hipGraphDestroy(g);
+    }
+    return 0;
+}
+
+// Advance by one unit of work without waiting: a graph replay of gper PC
+// iterations when that is allowed and at least gper steps remain, else one
+// iteration issued kernel by kernel.  *done = steps advanced.  traj_row, when
+// set, records the impurity after an eagerly issued step (graphs are used
+// only without a trajectory).
+static int evolve_unit(hetd_ctx* c, double dt, int remaining, double* traj_row, int* done) {
+    *done = 1;
+    c->iter_since_set++;
+    if (c->iter_since_set <= 3) {
+        // errHe/errimp are zero for RK steps (the driver prints them only after PC)
+        HCHK(c, hipMemsetAsync(&c->imp->errHe, 0, 2 * sizeof(double), c->st));
+        int rc = steprk_dev(c, dt, true);
+        if (rc) return rc;
+        rc = potential_update(c, true);
+        if (rc) return rc;
+    } else {
+        const bool graphs = c->use_graphs && !traj_row && c->tclass == 0 && c->iter_since_set > 4;
+        if (graphs && (!c->gexec || c->gdt != dt)) {
+            int rc = build_step_graph(c, dt);
+            if (rc) return rc;
+        }
+        if (graphs && c->gexec && remaining >= c->gper && step_sig(c) == c->gsig) {
+            HCHK(c, hipGraphLaunch(c->gexec, c->st));
+            c->iter_since_set += c->gper - 1;
+            *done = c->gper;
+            return 0;
+        }
+        int rc = pc_iteration(c, dt);
+        if (rc) return rc;
+    }
+    if (traj_row) {
+        HCHK(c, launch_traj_record(c->imp, traj_row, c->st));
+        if (c->force_pending) c->pend_rec = traj_row + 8;  // aimp, filled when the force is formed
+    }
+    return 0;
+}
+
 // --------------------------------------------------------------------- ABI
 
 extern "C" {
@@ -659,6 +793,7 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
         c->ax[a].n = nn[a];
     }
     HCHK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    if (const char* e = getenv("HETD_GRAPHS")) c->use_graphs = e[0] != '0';
     // grid tables (DFT4He3d.f90:309-317, 689-699)
     const double hh[3] = {c->hx, c->hy, c->hz}, mx[3] = {p->xmax, p->ymax, p->zmax};
     const double tmean[3] = {p->txmean, p->tymean, p->tzmean}, tsurf[3] = {p->txsurf, p->tysurf, p->tzsurf};
@@ -786,6 +921,7 @@ int hetd_finalize(hetd_ctx* c) {
     if (c->dtmp[0]) (void)hipFree(c->dtmp[0]);
     if (c->dtmp[1]) (void)hipFree(c->dtmp[1]);
     for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return HETD_OK;
@@ -900,31 +1036,44 @@ int hetd_evolve(hetd_ctx* c, double dt, int32_t nsteps, double* traj) {
         HCHK(c, hipMalloc(&c->traj, sizeof(double) * (size_t)nsteps * HETD_TRAJ_STRIDE));
         c->traj_cap = (long)nsteps * HETD_TRAJ_STRIDE;
     }
-    for (int s = 0; s < nsteps; ++s) {
-        int rc;
-        c->iter_since_set++;
-        if (c->iter_since_set <= 3) {
-            // errHe/errimp are zero for RK steps (the driver prints them only after PC)
-            HCHK(c, hipMemsetAsync(&c->imp->errHe, 0, 2 * sizeof(double), c->st));
-            rc = steprk_dev(c, dt, true);
-        } else {
-            rc = steppc_dev(c, dt, true);
-        }
+    for (int s = 0; s < nsteps;) {
+        int done = 0;
+        const int rc = evolve_unit(c, dt, nsteps - s, traj ? c->traj + (long)s * HETD_TRAJ_STRIDE : nullptr, &done);
         if (rc) return rc;
-        // driver: potenimp(rimp); poten(); forceimp(rimp,aimp); aimp/=m (DFT4He3d.f90:849-852)
-        rc = potential_update(c, true);
-        if (rc) return rc;
-        if (traj) {
-            double* row = c->traj + (long)s * HETD_TRAJ_STRIDE;
-            HCHK(c, launch_traj_record(c->imp, row, c->st));
-            if (c->force_pending) c->pend_rec = row + 8;  // aimp, filled when the force is formed
-        }
+        s += done;
     }
     if (int rc = settle_force(c)) return rc;
     if (traj && nsteps > 0)
         HCHK(c, hipMemcpyAsync(traj, c->traj, sizeof(double) * (size_t)nsteps * HETD_TRAJ_STRIDE,
                                hipMemcpyDeviceToHost, c->st));
     return check_guard(c);
+}
+
+int hetd_pool_evolve(hetd_ctx* const* ctxs, int32_t n, double dt, int32_t nsteps) {
+    if (!ctxs || n < 0 || nsteps < 0) return HETD_ERR_ARG;
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) return HETD_ERR_ARG;
+        if (!ctxs[i]->have_state) return fail(ctxs[i], HETD_ERR_STATE, "no state");
+    }
+    // units of work are issued round-robin, so every member's stream always
+    // has queued work and the members run side by side on the GPU
+    std::vector<int> left(n, nsteps);
+    for (bool busy = true; busy;) {
+        busy = false;
+        for (int i = 0; i < n; ++i) {
+            if (left[i] <= 0) continue;
+            int done = 0;
+            const int rc = evolve_unit(ctxs[i], dt, left[i], nullptr, &done);
+            if (rc) return rc;
+            left[i] -= done;
+            busy = true;
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        if (int rc = settle_force(ctxs[i])) return rc;
+    for (int i = 0; i < n; ++i)
+        if (int rc = check_guard(ctxs[i])) return rc;
+    return HETD_OK;
 }
 
 int hetd_energy(hetd_ctx* c, hetd_energies* e) {

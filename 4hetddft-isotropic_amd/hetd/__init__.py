@@ -188,3 +188,17 @@ class Context:
         ov = out.view(np.float64) if cplx else out
         self._chk(self.L.hetd_deriv(self.h, id_, iv, int(cplx), _p(fv), _p(ov)))
         return out
+
+
+def pool_evolve(contexts, dt, nsteps):
+    """Advance every Context in `contexts` by nsteps iterations (hetd_pool_evolve):
+    the same states as calling ctx.evolve(dt, nsteps, traj=False) on each, with
+    the members sharing the GPU."""
+    if not contexts:
+        return
+    L = contexts[0].L
+    arr = (C.c_void_p * len(contexts))(*[c.h.value for c in contexts])
+    rc = L.hetd_pool_evolve(arr, len(contexts), dt, nsteps)
+    if rc != 0:
+        msgs = [c.L.hetd_last_error(c.h).decode() for c in contexts]
+        raise RuntimeError("hetd_pool_evolve error %d: %s" % (rc, "; ".join(m for m in msgs if m)))

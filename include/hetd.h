@@ -147,6 +147,15 @@ int hetd_norm(hetd_ctx *ctx, double *norm);
  * receives nsteps*HETD_TRAJ_STRIDE doubles. */
 int hetd_evolve(hetd_ctx *ctx, double deltat, int32_t nsteps, double *traj);
 
+/* Ensemble of independent runs (no reference counterpart: the reference
+ * runs one droplet per process).  Advances each of the n contexts by nsteps
+ * iterations exactly as hetd_evolve(ctx[i], deltat, nsteps, NULL) would --
+ * bitwise the same states -- with the members' launches interleaved over
+ * their own streams so that they share the GPU, and steady-state
+ * predictor-corrector iterations replayed from per-member HIP graphs.
+ * Contexts may differ in grid, precision and parameters. */
+int hetd_pool_evolve(hetd_ctx *const *ctx, int32_t n, double deltat, int32_t nsteps);
+
 /* Copy a field to / from host memory (HETD_F_*; complex fields are
  * 2*nx*ny*nz doubles, half spectra (nx/2+1)*ny*nz doubles). */
 int hetd_get_field(hetd_ctx *ctx, int32_t which, double *host);

@@ -1,0 +1,78 @@
+"""Graph-replayed steps and the ensemble entry point against eager launches.
+
+hetd_evolve without a trajectory replays steady-state predictor-corrector
+iterations from a captured HIP graph; hetd_pool_evolve interleaves several
+contexts.  Both must give bitwise the states of the eager, one-context path
+(which the parity tests hold to the oracle).  Replays are checked several
+times in a row, with eager work (energy, norm, field copies) in between."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import golden_util as G
+from hetd import Context, capi, pool_evolve
+from hetd.initstate import woods_saxon_state
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make(shape, prec, icon, graphs, n4=150, seed=3, r0=(0.3, -0.2, 0.1), v0=(1.0, 2.0, -3.0)):
+    nz, ny, nx = shape
+    h = 0.6
+    p = capi.default_params(nx=nx, ny=ny, nz=nz, xmax=nx * h / 2, ymax=ny * h / 2, zmax=nz * h / 2, npd=13,
+                            icon=icon, precision=prec)
+    os.environ["HETD_GRAPHS"] = "1" if graphs else "0"
+    try:
+        ctx = Context(p, selec="Xe_He", r_cutoff=2.55, umax=2793.9365463)
+    finally:
+        del os.environ["HETD_GRAPHS"]
+    ctx.set_state(woods_saxon_state(nx, ny, nz, p.xmax, p.ymax, p.zmax, n4, seed), r0, v0)
+    ctx.init()
+    return ctx
+
+
+def same_state(a, b):
+    assert np.array_equal(a.field(capi.F_PSI), b.field(capi.F_PSI))
+    for x, y in zip(a.imp(), b.imp()):
+        assert np.array_equal(x, y)
+    ea, eb = a.energy(), b.energy()
+    for k in ea:
+        assert ea[k] == eb[k], k
+
+
+@pytest.mark.parametrize("shape,prec,icon", [((48, 40, 36), capi.HETD_FP64, 13),
+                                             ((32, 32, 32), capi.HETD_MIXED, 8)])
+def test_graph_replays_match_eager(shape, prec, icon):
+    say("graphs", shape, prec, icon)
+    eager = make(shape, prec, icon, graphs=False)
+    graph = make(shape, prec, icon, graphs=True)
+    for n in (9, 14, 6, 25, 12):
+        eager.evolve(G.DT, n, traj=False)
+        graph.evolve(G.DT, n, traj=False)
+        assert eager.norm() == graph.norm()
+        same_state(eager, graph)
+    eager.close()
+    graph.close()
+
+
+def test_pool_matches_sequential():
+    specs = [((32, 32, 32), capi.HETD_FP64, 13), ((40, 36, 48), capi.HETD_MIXED, 13),
+             ((30, 32, 45), capi.HETD_FP64, 8)]
+    say("pool", specs)
+    seq = [make(s, p, i, graphs=True, seed=5 + k) for k, (s, p, i) in enumerate(specs)]
+    pool = [make(s, p, i, graphs=True, seed=5 + k) for k, (s, p, i) in enumerate(specs)]
+    for n in (17, 8):
+        for c in seq:
+            c.evolve(G.DT, n, traj=False)
+        pool_evolve(pool, G.DT, n)
+        for a, b in zip(seq, pool):
+            same_state(a, b)
+    for c in seq + pool:
+        c.close()
