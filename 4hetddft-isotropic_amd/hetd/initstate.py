@@ -126,3 +126,42 @@ def read_mode2(path):
     psi = (c[0::2] + 1j * c[1::2]).reshape(nz, ny, nx)
     grid = dict(nx=nx, ny=ny, nz=nz, xmax=xmax, ymax=ymax, zmax=zmax, hx=hx, hy=hy, hz=hz)
     return psi, grid, rimp, vimp
+
+
+def smooth_sizes(lo, hi):
+    """Even grid edges in [lo, hi] with no prime factor above 5 (what the FFT
+    passes accept), ascending."""
+    out = []
+    for n in range(lo + (lo & 1), hi + 1, 2):
+        m = n
+        for q in (2, 3, 5):
+            while m % q == 0:
+                m //= q
+        if m == 1:
+            out.append(n)
+    return out
+
+
+def ensemble_members(nruns=32, seed0=100, spacing=0.4, nmin=64, nmax=192):
+    """Independent droplet + impurity runs of uneven size: per seed, He number
+    N uniform in 200..3000, a cubic grid of the smallest FFT-friendly edge
+    >= (2R + 16 A)/spacing (R = 2.22 N^(1/3)) clamped to [nmin, nmax], and an
+    impurity at the centre moving at 0.5..4.0 A/ps in a uniform random
+    direction.  Returns dicts n4, n, xmax, seed, v (A/ps)."""
+    sizes = smooth_sizes(nmin, nmax)
+    out = []
+    for k in range(nruns):
+        seed = seed0 + k
+        u = splitmix64_uniform(seed, 4)  # uniform on [-1, 1)
+        w = 0.5 * (u + 1.0)               # uniform on [0, 1)
+        n4 = 200 + int(w[0] * 2801)
+        radius = 2.22 * n4 ** (1.0 / 3.0)
+        need = (2.0 * radius + 16.0) / spacing
+        n = next((s for s in sizes if s >= need), sizes[-1])
+        speed = 0.5 + 3.5 * w[1]
+        cz = 2.0 * w[2] - 1.0
+        ph = 2.0 * np.pi * w[3]
+        sz = np.sqrt(max(0.0, 1.0 - cz * cz))
+        v = (speed * sz * np.cos(ph), speed * sz * np.sin(ph), speed * cz)
+        out.append({"n4": n4, "n": n, "xmax": 0.5 * n * spacing, "seed": seed, "v": v})
+    return out

@@ -193,6 +193,50 @@ def accuracy_run(args, prec, psi):
     return {"e0": e0, "e1": e1, "norm": norm, "traj": traj, "r": r, "v": v}
 
 
+def ensemble_leg(args):
+    """Batched ensemble of independent droplet + impurity runs of uneven size
+    (mixed precision): aggregate throughput of hetd_pool_evolve against the
+    same runs advanced one after another with hetd_evolve."""
+    from hetd import Context, capi, pool_evolve
+    from hetd.initstate import ensemble_members, woods_saxon_state
+    members = ensemble_members(args.ensemble_runs)
+    dt = 5e-4 / KTOPS
+    ctxs = []
+    t0 = time.perf_counter()
+    for m in members:
+        n, xm = m["n"], m["xmax"]
+        p = capi.default_params(nx=n, ny=n, nz=n, xmax=xm, ymax=xm, zmax=xm, npd=13, icon=13,
+                                precision=capi.HETD_MIXED)
+        c = Context(p, selec="Xe_He", r_cutoff=2.55, umax=2793.9365463)
+        c.set_state(woods_saxon_state(n, n, n, xm, xm, xm, m["n4"], m["seed"]), (0.0, 0.0, 0.0),
+                    tuple(v * KTOPS for v in m["v"]))
+        c.init()
+        ctxs.append(c)
+    setup_s = time.perf_counter() - t0
+    pool_evolve(ctxs, dt, 6)  # Runge-Kutta start and the first PC steps
+    k = args.ensemble_steps
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.evolve(dt, k, traj=False)
+    seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pool_evolve(ctxs, dt, k)
+    pool = time.perf_counter() - t0
+    pts = sum(float(m["n"]) ** 3 for m in members)
+    norms = [c.norm() / m["n4"] for c, m in zip(ctxs, members)]
+    for c in ctxs:
+        c.close()
+    return {"runs": len(members), "steps_timed": k, "precision": "mixed",
+            "grids": sorted(set(m["n"] for m in members)), "total_points": int(pts),
+            "sequential_Mpoint_steps_per_s": round(pts * k / seq / 1e6, 1),
+            "pool_Mpoint_steps_per_s": round(pts * k / pool / 1e6, 1),
+            "pool_over_sequential": round(seq / pool, 3),
+            "sequential_s": round(seq, 3), "pool_s": round(pool, 3), "setup_s": round(setup_s, 1),
+            "norm_per_atom_range": [min(norms), max(norms)],
+            "workload": "%d He-N + Xe runs, N uniform 200..3000 (seeds 100..), cubic grids 64..192 at 0.4 A, "
+                        "impurity 0.5..4 A/ps in random directions, mixed precision, npd 13" % len(members)}
+
+
 def compare_accuracy(a64, amx):
     """Mixed against FP64 after the same steps: relative energy errors (total
     and per component scale), norm, impurity position / velocity."""
@@ -223,6 +267,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run device copies of MIB MiB (the byte reference for PMC traffic counters)")
+    ap.add_argument("--ensemble-runs", type=int, default=32)
+    ap.add_argument("--ensemble-steps", type=int, default=60,
+                    help="timed steps of the batched-ensemble leg (0: skip)")
     ap.add_argument("--check-steps", type=int, default=400,
                     help="untimed mixed-vs-FP64 accuracy run length (0: skip)")
     args = ap.parse_args()
@@ -250,6 +297,7 @@ def main():
     if len(res) == 2 and args.check_steps > 0:
         acc = compare_accuracy(accuracy_run(args, capi.HETD_FP64, psi), accuracy_run(args, capi.HETD_MIXED, psi))
     del psi
+    ens = ensemble_leg(args) if args.ensemble_steps > 0 and args.ensemble_runs > 0 else None
     head = "mixed" if "mixed" in res else "fp64"
     r = res[head]
     pts = float(n) ** 3
@@ -298,7 +346,7 @@ def main():
         "data": "synthetic (Woods-Saxon He-1000 droplet, seeded phase, Xe at the centre moving 200 m/s)",
         "config": {"workload": "He-1000+Xe %d^3 0.4A OT+solid npd13 icon13 dt0.5fs predictor-corrector steps" % n,
                    "grid": [n, n, n], "precision": head, "wall_ms_per_step": round(r["wall"] * 1e3 / args.steps, 4),
-                   "per_precision": per_prec, "mixed_vs_fp64": cmp_},
+                   "per_precision": per_prec, "mixed_vs_fp64": cmp_, "ensemble": ens},
         "roofline": {"bound": "hbm", "kernel": args.kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4),
                      "traffic": traffic, "launches": r["klaunch"],
