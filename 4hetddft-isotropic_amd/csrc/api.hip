@@ -152,7 +152,9 @@ template <typename T> static hipError_t run_x(hetd_ctx* c, XPassArgs<T> x) {
     default: b = 2 * s * nh + 4 * s * n + (x.rout2 ? s * n : 0); break;
     }
     tbeg(c, HETD_K_XPASS);
-    hipError_t e = reg_fft_supported(x.ax.n) ? launch_xreg(x, c->st) : launch_xpass(x, c->st);
+    hipError_t e = reg_fft_supported(x.ax.n) ? launch_xreg(x, c->st)
+                   : reg2_supported(x.ax.n)  ? launch_x2(x, c->st)
+                                             : launch_xpass(x, c->st);
     tend(c, HETD_K_XPASS, b);
     return e;
 }
@@ -160,7 +162,9 @@ template <typename T> static hipError_t run_x(hetd_ctx* c, XPassArgs<T> x) {
 template <typename T> static hipError_t run_y(hetd_ctx* c, YPassArgs<T> y, int nbuf) {
     y.ldh = c->ldh;
     tbeg(c, HETD_K_YPASS);
-    hipError_t e = reg_fft_supported(y.ax.n) ? launch_yreg(y, c->nz, nbuf, c->st) : launch_ypass(y, c->nz, nbuf, c->st);
+    hipError_t e = reg_fft_supported(y.ax.n) ? launch_yreg(y, c->nz, nbuf, c->st)
+                   : reg2_supported(y.ax.n)  ? launch_y2(y, c->nz, nbuf, c->st)
+                                             : launch_ypass(y, c->nz, nbuf, c->st);
     tend(c, HETD_K_YPASS, 4.0 * sizeof(T) * c->nh * nbuf);
     return e;
 }
@@ -172,7 +176,10 @@ template <typename T> static hipError_t run_z(hetd_ctx* c, ZPassArgs<T> z, int n
     if (z.mode == ZMODE_CONV)
         per = 2 * s + 3 * s * z.nk + (z.fwd_out ? 2 * s : 0.0) + (z.kout ? 3 * s : 0.0) + (z.addin ? 2 * s : 0.0);
     tbeg(c, HETD_K_ZPASS);
-    hipError_t e = reg_fft_supported(z.ax.n) ? launch_zreg(z, nbuf, c->st) : launch_zpass(z, nbuf, c->st);
+    const bool reg2 = reg2_supported(z.ax.n) && (z.mode != ZMODE_CONV || z.nk == 1);
+    hipError_t e = reg_fft_supported(z.ax.n) ? launch_zreg(z, nbuf, c->st)
+                   : reg2                    ? launch_z2(z, nbuf, c->st)
+                                             : launch_zpass(z, nbuf, c->st);
     tend(c, HETD_K_ZPASS, per * c->nh * nbuf);
     return e;
 }

@@ -55,6 +55,11 @@ bool reg_fft_supported(int n);
 template <typename T> hipError_t launch_xreg(XPassArgs<T> a, hipStream_t st);
 template <typename T> hipError_t launch_yreg(YPassArgs<T> a, int nz, int nbuf, hipStream_t st);
 template <typename T> hipError_t launch_zreg(ZPassArgs<T> a, int nbuf, hipStream_t st);
+// register-resident x/y/z passes for N = P*Q (fft_reg2.hip); z: FWD, INV, CONV with nk = 1
+bool reg2_supported(int n);
+template <typename T> hipError_t launch_x2(XPassArgs<T> a, hipStream_t st);
+template <typename T> hipError_t launch_y2(YPassArgs<T> a, int nz, int nbuf, hipStream_t st);
+template <typename T> hipError_t launch_z2(ZPassArgs<T> a, int nbuf, hipStream_t st);
 
 // --------------------------------------------------------------- grid info
 

@@ -27,7 +27,8 @@ def make_pair(oracle_mod, p, selec="null", r_cutoff=2.0, umax=0.0):
 
 @pytest.mark.parametrize("shape", [(8, 6, 10), (27, 10, 12), (16, 16, 16), (5, 9, 45), (20, 12, 36),
                                    (32, 24, 64), (40, 30, 50), (16, 64, 256), (256, 16, 64), (64, 256, 16),
-                                   (15, 9, 256)])
+                                   (15, 9, 256), (384, 6, 10), (6, 384, 12), (8, 6, 384), (162, 20, 216),
+                                   (12, 288, 320), (192, 180, 150)])
 def test_fft_matches_numpy_and_oracle(oracle_mod, shape):
     say("fft", shape)
     nz, ny, nx = shape
