@@ -115,6 +115,12 @@ module hetd_capi
       integer(c_int32_t), value :: nsteps
       real(c_double), dimension(*), intent(out) :: traj
     end function
+    integer(c_int) function hetd_pool_evolve(ctxs, n, dt, nsteps) bind(C, name='hetd_pool_evolve')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), dimension(*), intent(in) :: ctxs
+      integer(c_int32_t), value :: n, nsteps
+      real(c_double), value :: dt
+    end function
     integer(c_int) function hetd_get_field(ctx, which, host) bind(C, name='hetd_get_field')
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctx
