@@ -15,6 +15,8 @@
 //   FORCE:  gradient of den (derden.f90:26-28) times u_imp, summed
 //           (potenimp.f90:44-46);
 //   ENERGY: gradient of psi and the energy densities (energy.f90:51-110).
+#include <type_traits>
+
 #include "hetd_kernels.h"
 
 namespace hetd {
@@ -78,14 +80,8 @@ struct TiledCommon {
 __device__ __forceinline__ double dens(double2 v) { return abs2(v); }
 
 // RHS history hpsiold: complex128, or complex64 in mixed precision
-template <bool MIX> __device__ __forceinline__ double2 get_hist(const double2* h, long o) {
-    if constexpr (MIX) {
-        const float2 v = reinterpret_cast<const float2*>(h)[o];
-        return make_double2((double)v.x, (double)v.y);
-    } else {
-        return h[o];
-    }
-}
+__device__ __forceinline__ double2 widen2(float2 v) { return make_double2((double)v.x, (double)v.y); }
+__device__ __forceinline__ double2 widen2(double2 v) { return v; }
 template <bool MIX> __device__ __forceinline__ void put_hist(double2* h, long o, double2 v) {
     if constexpr (MIX) reinterpret_cast<float2*>(h)[o] = make_float2((float)v.x, (float)v.y);
     else h[o] = v;
@@ -101,7 +97,8 @@ template <bool MIX> __device__ __forceinline__ void put_den(const UpdateArgs& a,
 // psi (LDS plane and z queue) and grad(den) * u_imp summed, which is
 // forceimp (potenimp.f90:24-55) of the state the update starts from.
 // MIX (UPDATE only): FP32 mean field in, FP32 den and complex64 RHS history.
-template <int KIND, int M, int TPY, int TNT, bool PF, bool FRC, bool MIX>
+// SK (UPDATE only): the stage, STAGE_RK / STAGE_PRED / STAGE_CORR.
+template <int KIND, int M, int TPY, int TNT, bool PF, bool FRC, bool MIX, int SK>
 __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KIND>::A args) {
     using V = typename TArgs<KIND>::V;
     constexpr int NC = TArgs<KIND>::NC;
@@ -236,6 +233,37 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
     }
     // software pipeline: the halo of plane iz and the queue fronts are
     // loaded one iteration ahead, while plane iz-1 is being computed
+    // UPDATE operands of a point (pot4, and q / psiold / hpsiold / pc by stage),
+    // loaded one plane ahead with the pipelined variants
+    using HT = typename std::conditional<MIX, float2, double2>::type;  // RHS history
+    using PT = typename std::conditional<MIX, float, double>::type;    // mean field
+    double2 opA[TPY], opB[TPY], opD[TPY];
+    HT opH1[TPY], opH2[TPY];
+    PT opP[TPY];
+    auto load_ops = [&](int p, long o) {
+        if constexpr (KIND == TK_UPDATE) {
+            const UpdateArgs& a = args;
+            if constexpr (MIX) opP[p] = a.pot4f[o];
+            else opP[p] = a.pot4[o];
+            if constexpr (SK == STAGE_RK) {
+                opA[p] = a.q[o];
+            } else {
+                opA[p] = a.po3[o];
+                opD[p] = a.pc[o];
+                opH1[p] = reinterpret_cast<const HT*>(a.h1)[o];
+                opH2[p] = reinterpret_cast<const HT*>(a.h2)[o];
+                if constexpr (SK == STAGE_CORR) opB[p] = a.po2[o];
+            }
+        }
+    };
+    constexpr bool OPF = PF && KIND == TK_UPDATE;
+    // (unconditional: an invalid point's c0 is an in-box column, and a load
+    // under a lane-dependent branch would be waited for at the branch end)
+    if constexpr (OPF)
+#pragma unroll
+        for (int p = 0; p < TPY; ++p)
+            if (iz0 < izend) load_ops(p, c0[p] + pl * iz0);
+
     V hraw[KH], qraw[TPY];  // prefetched raw values; modes hm[] and qm[]
     int qm[TPY];
 #pragma unroll
@@ -334,6 +362,7 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
             const double u = uimp_at(g.potion, g.npot, g.delinter, g.x[ix] - R[0], dy * dy + dz * dz);
             if constexpr (KIND == TK_UPDATE) {
                 const UpdateArgs& a = args;
+                if constexpr (!OPF) load_ops(p, o);
                 if constexpr (FRC) {
                     // grad den of the input state, as the FORCE kind forms it
                     double gx = 0.0, gy = 0.0, gz = 0.0;
@@ -358,9 +387,7 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                 }
                 const double lr = (sx.x + sy.x) + sz.x, li = (sx.y + sy.y) + sz.y;
                 const double2 ps = q[p][M];
-                double pot;
-                if constexpr (MIX) pot = (double)a.pot4f[o];
-                else pot = a.pot4[o];
+                const double pot = (double)opP[p];
                 const double tr = g.lambdah * ((((((1.0 + g.tcx[ix]) + 1.0) + g.tcy[iy[p]]) + 1.0) + g.tcz[iz]));
                 const double ar = lr * a.h2o2m4 - pot * ps.x;
                 const double ai = li * a.h2o2m4 - pot * ps.y;
@@ -368,8 +395,8 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                 fv.x = (tr * ar - ai) + u * ps.y;
                 fv.y = (tr * ai + ar) - u * ps.x;
                 double2 pn;
-                if (a.kind == STAGE_RK) {
-                    const double2 qq = a.q[o];
+                if constexpr (SK == STAGE_RK) {
+                    const double2 qq = opA[p];
                     double2 s1, qn;
                     s1.x = a.ra * (fv.x - a.rb * qq.x);
                     s1.y = a.ra * (fv.y - a.rb * qq.y);
@@ -383,10 +410,10 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     a.psi_out[o] = pn;
                     put_den<MIX>(a, o, d);
                     if (!(d > 0.0)) atomicOr(a.guard, 1);
-                } else if (a.kind == STAGE_PRED) {
+                } else if constexpr (SK == STAGE_PRED) {
                     const double c4 = (4.0 / 3.0) * a.dt;
-                    const double2 p3 = a.po3[o], h1 = get_hist<MIX>(a.h1, o), h2 = get_hist<MIX>(a.h2, o);
-                    const double2 pc = a.pc[o];
+                    const double2 p3 = opA[p], h1 = widen2(opH1[p]), h2 = widen2(opH2[p]);
+                    const double2 pc = opD[p];
                     double2 P;
                     P.x = p3.x + c4 * ((2.0 * fv.x - h1.x) + 2.0 * h2.x);
                     P.y = p3.y + c4 * ((2.0 * fv.y - h1.y) + 2.0 * h2.y);
@@ -399,8 +426,8 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     const double d = abs2(pn);
                     put_den<MIX>(a, o, d);
                 } else {
-                    const double2 p3 = a.po3[o], p2 = a.po2[o], pc = a.pc[o];
-                    const double2 h1 = get_hist<MIX>(a.h1, o), h2 = get_hist<MIX>(a.h2, o);
+                    const double2 p3 = opA[p], p2 = opB[p], pc = opD[p];
+                    const double2 h1 = widen2(opH1[p]), h2 = widen2(opH2[p]);
                     const double t3 = 3.0 * a.dt;
                     double2 Cc, pcn;
                     Cc.x = 0.125 * ((9.0 * p3.x - p2.x) + t3 * ((fv.x + 2.0 * h1.x) - h2.x));
@@ -436,6 +463,11 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                 acc[EN_NORM] += d;
             }
         }
+        // the next plane's update operands, once this plane's are consumed
+        if constexpr (OPF)
+            if (iz + 1 < izend)
+#pragma unroll
+                for (int p = 0; p < TPY; ++p) load_ops(p, c0[p] + pl * (iz + 1));
         // shift the z queues
 #pragma unroll
         for (int p = 0; p < TPY; ++p)
@@ -446,7 +478,7 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
             }
     }
     bool reduce = true;
-    if constexpr (KIND == TK_UPDATE) reduce = FRC || args.kind == STAGE_CORR;
+    if constexpr (KIND == TK_UPDATE) reduce = FRC || SK == STAGE_CORR;
     if (reduce) {
         block_sum<NC>(acc, red);
         if (threadIdx.x == 0) {
@@ -492,23 +524,36 @@ bool tiled_supported(const StencilCoef& sc) { return sc.m == 4 || sc.m == 6; }
 // threads (64x4 tiles, several workgroups per CU, so one workgroup's loads
 // overlap another's arithmetic) with the next plane's loads issued before
 // the current plane is computed beat 512-thread 64x8 tiles by 0.15 ms/step.
-// HETD_TILE_<KIND>=0..3 selects another variant.
+// HETD_TILE_FORCE / HETD_TILE_ENERGY = 0..3 select another variant for those
+// kinds; the update kernels (one per stage) are built for variant 3 only.
 struct TileVariant { int tpy, tnt; bool pf; };
 static constexpr int kNVariants = 4;
 static constexpr TileVariant kVariants[kNVariants] = {{1, 512, false}, {1, 256, false}, {1, 512, true}, {1, 256, true}};
 
+static constexpr int kUpdateVariant = 3;  // the update kernels are built for this one only
+
 static int variant_for(int kind) {
-    static const char* names[3] = {"HETD_TILE_UPDATE", "HETD_TILE_FORCE", "HETD_TILE_ENERGY"};
+    if (kind == TK_UPDATE) return kUpdateVariant;
+    static const char* names[3] = {"", "HETD_TILE_FORCE", "HETD_TILE_ENERGY"};
     const char* e = getenv(names[kind]);
     if (e && e[0] >= '0' && e[0] < '0' + kNVariants) return e[0] - '0';
-    return kind == TK_UPDATE ? 3 : 0;
+    return 0;
 }
 
-template <int KIND, int M, int V, bool FRC, bool MIX>
+template <int KIND, int M, int V, bool FRC, bool MIX, int SK>
 static void launch_k(const TiledCommon& tc, const typename TArgs<KIND>::A& a, int nlaunch, hipStream_t st) {
     constexpr TileVariant tv = kVariants[V];
-    hipLaunchKernelGGL((k_tiled<KIND, M, tv.tpy, tv.tnt, tv.pf, FRC, MIX>), dim3(nlaunch), dim3(tv.tnt), 0, st, tc,
-                       a);
+    hipLaunchKernelGGL((k_tiled<KIND, M, tv.tpy, tv.tnt, tv.pf, FRC, MIX, SK>), dim3(nlaunch), dim3(tv.tnt), 0, st,
+                       tc, a);
+}
+
+template <int M, int V, bool FRC, bool MIX>
+static void launch_stage(const TiledCommon& tc, const UpdateArgs& a, int nlaunch, hipStream_t st) {
+    switch (a.kind) {
+    case STAGE_RK: launch_k<TK_UPDATE, M, V, FRC, MIX, STAGE_RK>(tc, a, nlaunch, st); break;
+    case STAGE_PRED: launch_k<TK_UPDATE, M, V, FRC, MIX, STAGE_PRED>(tc, a, nlaunch, st); break;
+    default: launch_k<TK_UPDATE, M, V, FRC, MIX, STAGE_CORR>(tc, a, nlaunch, st); break;
+    }
 }
 
 template <int KIND, int M, int V>
@@ -516,24 +561,28 @@ static void launch_v(const TiledCommon& tc, const typename TArgs<KIND>::A& a, in
     if constexpr (KIND == TK_UPDATE) {
         const bool mix = a.denf != nullptr;
         if (a.force) {
-            if (mix) launch_k<KIND, M, V, true, true>(tc, a, nlaunch, st);
-            else launch_k<KIND, M, V, true, false>(tc, a, nlaunch, st);
+            if (mix) launch_stage<M, V, true, true>(tc, a, nlaunch, st);
+            else launch_stage<M, V, true, false>(tc, a, nlaunch, st);
         } else {
-            if (mix) launch_k<KIND, M, V, false, true>(tc, a, nlaunch, st);
-            else launch_k<KIND, M, V, false, false>(tc, a, nlaunch, st);
+            if (mix) launch_stage<M, V, false, true>(tc, a, nlaunch, st);
+            else launch_stage<M, V, false, false>(tc, a, nlaunch, st);
         }
     } else {
-        launch_k<KIND, M, V, false, false>(tc, a, nlaunch, st);
+        launch_k<KIND, M, V, false, false, 0>(tc, a, nlaunch, st);
     }
 }
 
 template <int KIND, int M>
 static void launch_m(int v, const TiledCommon& tc, const typename TArgs<KIND>::A& a, int nlaunch, hipStream_t st) {
-    switch (v) {
-    case 0: launch_v<KIND, M, 0>(tc, a, nlaunch, st); break;
-    case 1: launch_v<KIND, M, 1>(tc, a, nlaunch, st); break;
-    case 2: launch_v<KIND, M, 2>(tc, a, nlaunch, st); break;
-    default: launch_v<KIND, M, 3>(tc, a, nlaunch, st); break;
+    if constexpr (KIND == TK_UPDATE) {
+        launch_v<KIND, M, kUpdateVariant>(tc, a, nlaunch, st);
+    } else {
+        switch (v) {
+        case 0: launch_v<KIND, M, 0>(tc, a, nlaunch, st); break;
+        case 1: launch_v<KIND, M, 1>(tc, a, nlaunch, st); break;
+        case 2: launch_v<KIND, M, 2>(tc, a, nlaunch, st); break;
+        default: launch_v<KIND, M, 3>(tc, a, nlaunch, st); break;
+        }
     }
 }
 
