@@ -943,12 +943,14 @@ int hetd_set_potion(hetd_ctx* c, const double* potion, int32_t npot, double deli
     c->npot = npot;
     c->delinter = delinter;
     fill_grid(c);
+    drop_graph(c);  // the captured launches hold the old grid arguments
     return HETD_OK;
 }
 
 int hetd_set_state(hetd_ctx* c, const double* psi, const double r[3], const double v[3]) {
     if (!c || !psi || !r || !v) return HETD_ERR_ARG;
     if (!c->potion) return fail(c, HETD_ERR_STATE, "hetd_set_potion must precede hetd_set_state");
+    drop_graph(c);
     const size_t nC = sizeof(double2) * c->n;
     HCHK(c, hipMemcpyAsync(c->psi, psi, nC, hipMemcpyHostToDevice, c->st));
     HCHK(c, launch_den_from_psi(c->psi, c->den, c->n, c->st));
