@@ -50,6 +50,7 @@ struct hetd_ctx {
     // (fused), before anything reads the acceleration
     bool force_pending = false;
     bool den64_ok = true;  // mixed: the FP64 den array matches psi (updates write only FP32 den)
+    bool dencg_ok = true;  // dencg holds dencg (false: the local part of pot4, see poten_t)
     double* pend_rec = nullptr;  // trajectory row waiting for that acceleration
     float *den32 = nullptr, *pot4_32 = nullptr, *dencg32 = nullptr, *delj4_32 = nullptr, *tmp32 = nullptr;
     float2 *A32 = nullptr, *B32 = nullptr, *C32 = nullptr;
@@ -378,10 +379,17 @@ static int poten_t(hetd_ctx* c, const PF<T>& f, bool full, bool want_ccor, bool 
     }
     c->delj4_state = full ? DELJ4_OK : DELJ4_SPEC;
     // dencg = c2r(fden*W)*renor, s = den*dencg*(cp4+cpp4*dencg), A = x-r2c(s)
+    // (stepping form: the local part of pot4 is formed here, where den is
+    // read anyway, and stored in place of dencg)
     XPassArgs<T> xc{};
     xc.ax = c->ax[0]; xc.nlines = nl; xc.mode = XMODE_CG_S; xc.cin = f.A; xc.rin = f.den; xc.rout = f.dencg;
     xc.cout = f.A; xc.scale = (T)c->renor; xc.cp4 = (T)c->p.cp4; xc.cpp4 = (T)c->p.cpp4;
+    xc.fold = full ? 0 : 1;
+    xc.a0 = (T)(c->p.cp4 / 2.0); xc.a1 = (T)(c->p.cpp4 / 3.0);
+    xc.lsolid = c->p.lsolid && !c->otc;
+    xc.csol = (T)c->p.c_solid; xc.beta = (T)c->p.beta_solid; xc.den_m = (T)c->p.den_m;
     HCHK(c, run_x(c, xc));
+    c->dencg_ok = full;
     y.inv = 0; y.buf[0] = f.A;
     HCHK(c, run_y(c, y, 1));
     ZPassArgs<T> z2{};
@@ -399,7 +407,31 @@ static int poten_t(hetd_ctx* c, const PF<T>& f, bool full, bool want_ccor, bool 
     xp.a0 = (T)(c->p.cp4 / 2.0); xp.a1 = (T)(c->p.cpp4 / 3.0);
     xp.lsolid = c->p.lsolid && !c->otc;
     xp.csol = (T)c->p.c_solid; xp.beta = (T)c->p.beta_solid; xp.den_m = (T)c->p.den_m;
+    xp.fold = full ? 0 : 1;
     HCHK(c, run_x(c, xp));
+    return 0;
+}
+
+// dencg = c2r(W r2c(den)) * renor (poten.f90:27-38) from the current den,
+// for the energy and field copies after stepping-form potentials (which keep
+// the local part of pot4 in the dencg buffer)
+template <typename T> static int dencg_t(hetd_ctx* c, const PF<T>& f) {
+    const long nl = (long)c->ny * c->nz;
+    XPassArgs<T> x{};
+    x.ax = c->ax[0]; x.nlines = nl; x.mode = XMODE_R2C; x.rin = f.den; x.cout = f.A;
+    HCHK(c, run_x(c, x));
+    YPassArgs<T> y{};
+    y.ax = c->ax[1]; y.nxh = c->nxh; y.inv = 0; y.buf[0] = f.A;
+    HCHK(c, run_y(c, y, 1));
+    ZPassArgs<T> z{};
+    z.ax = c->ax[2]; z.nxh = c->nxh; z.ny = c->ny; z.mode = ZMODE_CONV; z.nk = 1; z.in[0] = f.A;
+    z.out[0][0] = f.A; z.kern[0] = f.wcgk;
+    HCHK(c, run_z(c, z, 1));
+    y.inv = 1;
+    HCHK(c, run_y(c, y, 1));
+    XPassArgs<T> xb{};
+    xb.ax = c->ax[0]; xb.nlines = nl; xb.mode = XMODE_C2R; xb.cin = f.A; xb.rout = f.dencg; xb.scale = (T)c->renor;
+    HCHK(c, run_x(c, xb));
     return 0;
 }
 
@@ -431,6 +463,18 @@ template <typename T> static int delj4_t(hetd_ctx* c, const PF<T>& f, bool from_
     return 0;
 }
 
+// (After a steppc called on its own, den has moved on from the middle poten;
+// the reference's energy() at that point would use the middle poten's dencg,
+// this one the current den's.  hetd_poten forms the full fields, and
+// hetd_evolve ends with a poten of the current den.)
+static int ensure_dencg(hetd_ctx* c) {
+    if (c->dencg_ok) return 0;
+    const int rc = c->mixed ? dencg_t<float>(c, pf32(c)) : dencg_t<double>(c, pf64(c));
+    if (rc) return rc;
+    c->dencg_ok = true;
+    return 0;
+}
+
 static int ensure_delj4(hetd_ctx* c) {
     if (c->delj4_state == DELJ4_OK) return 0;
     const bool from_b = c->delj4_state == DELJ4_SPEC;
@@ -440,9 +484,10 @@ static int ensure_delj4(hetd_ctx* c) {
     return 0;
 }
 
-static int poten_dev(hetd_ctx* c, bool want_ccor = false, bool want_fden = false, bool keep_pot4 = true) {
+static int poten_dev(hetd_ctx* c, bool want_ccor = false, bool want_fden = false, bool keep_pot4 = true,
+                     bool full_form = false) {
     // OTC keeps the full form: its K_alpha spectrum rides on the first z pass
-    const bool full = want_ccor || want_fden || c->otc;
+    const bool full = want_ccor || want_fden || c->otc || full_form;
     if (c->mixed) return poten_t<float>(c, pf32(c), full, want_ccor, want_fden, keep_pot4);
     int rc = poten_t<double>(c, pf64(c), full, want_ccor, want_fden, keep_pot4);
     if (rc) return rc;
@@ -611,6 +656,7 @@ static std::vector<long> step_sig(const hetd_ctx* c) {
     v.push_back(c->force_pending);
     v.push_back(c->delj4_state);
     v.push_back(c->den64_ok);
+    v.push_back(c->dencg_ok);
     return v;
 }
 
@@ -628,6 +674,7 @@ static void restore_sig(hetd_ctx* c, const std::vector<long>& v) {
     c->force_pending = v[k++] != 0;
     c->delj4_state = (int)v[k++];
     c->den64_ok = v[k++] != 0;
+    c->dencg_ok = v[k++] != 0;
 }
 
 // one steady-state iteration of the driver loop (DFT4He3d.f90:844-852)
@@ -986,7 +1033,8 @@ int hetd_potenimp(hetd_ctx* c, const double r[3]) {
 int hetd_poten(hetd_ctx* c) {
     if (!c) return HETD_ERR_ARG;
     if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
-    int rc = poten_dev(c, false, false);
+    // called on its own: the full form, so that every field of poten is formed
+    int rc = poten_dev(c, false, false, true, true);
     if (rc) return rc;
     HCHK(c, hipStreamSynchronize(c->st));
     return HETD_OK;
@@ -1089,6 +1137,7 @@ int hetd_energy(hetd_ctx* c, hetd_energies* e) {
     if (!c || !e) return HETD_ERR_ARG;
     if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
     if (int rc = ensure_delj4(c)) return rc;
+    if (int rc = ensure_dencg(c)) return rc;
     EnergyArgs a{};
     a.g = c->grid; a.sc = c->sc; a.psi = c->psi; a.den = c->den; a.dencg = c->dencg; a.delj4 = c->delj4;
     a.imp = c->imp; a.aux1 = 0.5 * c->p.cp4; a.aux2 = (1.0 / 3.0) * c->p.cpp4; a.beta = c->p.beta_solid;
@@ -1164,7 +1213,10 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
     case HETD_F_PSI: src = c->psi; bytes = nC; break;
     case HETD_F_DEN: src = c->den; break;
     case HETD_F_POT4: src = mx ? widen(c, c->pot4_32, c->n, c->tmp) : c->pot4; break;
-    case HETD_F_DENCG: src = mx ? widen(c, c->dencg32, c->n, c->tmp) : c->dencg; break;
+    case HETD_F_DENCG:
+        if ((rc = ensure_dencg(c))) return rc;
+        src = mx ? widen(c, c->dencg32, c->n, c->tmp) : c->dencg;
+        break;
     case HETD_F_DELJ4:
         if ((rc = ensure_delj4(c))) return rc;
         src = mx ? widen(c, c->delj4_32, c->n, c->tmp) : c->delj4;

@@ -71,32 +71,13 @@ __global__ void __launch_bounds__(256) k_xpass(XPassArgs<T> a) {
                 a.rout[ia] = va;
                 if (hb) a.rout[ib] = vb;
             } else if (a.mode == XMODE_CG_S) {
-                // dencg = c2r(fden*W)*renor (poten.f90:34-38), then the
-                // correlation source sto1 = den*dencg*(cp4+cpp4*dencg) (:62-64)
-                a.rout[ia] = va;
-                T sa = a.rin[ia] * va * (a.cp4 + a.cpp4 * va);
-                T sb = (T)0;
-                if (hb) {
-                    a.rout[ib] = vb;
-                    sb = a.rin[ib] * vb * (a.cp4 + a.cpp4 * vb);
-                }
+                // dencg (or the folded local part), then the correlation source
+                const T sa = cgs_point<T>(a, ia, va);
+                const T sb = hb ? cgs_point<T>(a, ib, vb) : (T)0;
                 lds[pp * nx + x] = cmk<C>(sa, sb);
-            } else {  // XMODE_POT: pot4 = delj4 + dencg^2 (a0+a1 dencg) + sto1 [+ penalty]
-                T ca = va, cb = vb;
-                {
-                    const T g = a.rin2[ia];
-                    T u = (a.rin3 ? a.rin3[ia] : (T)0) + g * g * (a.a0 + a.a1 * g) + ca;
-                    if (a.lsolid) u = u + solid_penalty<T>(a.rin[ia], a.csol, a.beta, a.den_m);
-                    if (a.rout) a.rout[ia] = u;
-                    if (a.rout2) a.rout2[ia] = ca;
-                }
-                if (hb) {
-                    const T g = a.rin2[ib];
-                    T u = (a.rin3 ? a.rin3[ib] : (T)0) + g * g * (a.a0 + a.a1 * g) + cb;
-                    if (a.lsolid) u = u + solid_penalty<T>(a.rin[ib], a.csol, a.beta, a.den_m);
-                    if (a.rout) a.rout[ib] = u;
-                    if (a.rout2) a.rout2[ib] = cb;
-                }
+            } else {  // XMODE_POT
+                pot_point<T>(a, ia, va);
+                if (hb) pot_point<T>(a, ib, vb);
             }
         }
         __syncthreads();

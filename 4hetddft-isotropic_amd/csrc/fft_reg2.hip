@@ -171,25 +171,12 @@ __global__ void __launch_bounds__((P > Q ? P : Q) * TL) k_x2(XPassArgs<T> a) {
                     if (va) a.rout[ia] = ra;
                     if (vb) a.rout[ib] = rb;
                 } else if (a.mode == XMODE_CG_S) {
-                    T sa = (T)0, sb = (T)0;
-                    if (va) { a.rout[ia] = ra; sa = a.rin[ia] * ra * (a.cp4 + a.cpp4 * ra); }
-                    if (vb) { a.rout[ib] = rb; sb = a.rin[ib] * rb * (a.cp4 + a.cpp4 * rb); }
+                    const T sa = va ? cgs_point<T>(a, ia, ra) : (T)0;
+                    const T sb = vb ? cgs_point<T>(a, ib, rb) : (T)0;
                     v[k] = cmk<C>(sa, sb);
                 } else {  // POT
-                    if (va) {
-                        const T g = a.rin2[ia];
-                        T u = (a.rin3 ? a.rin3[ia] : (T)0) + g * g * (a.a0 + a.a1 * g) + ra;
-                        if (a.lsolid) u = u + solid_penalty<T>(a.rin[ia], a.csol, a.beta, a.den_m);
-                        if (a.rout) a.rout[ia] = u;
-                        if (a.rout2) a.rout2[ia] = ra;
-                    }
-                    if (vb) {
-                        const T g = a.rin2[ib];
-                        T u = (a.rin3 ? a.rin3[ib] : (T)0) + g * g * (a.a0 + a.a1 * g) + rb;
-                        if (a.lsolid) u = u + solid_penalty<T>(a.rin[ib], a.csol, a.beta, a.den_m);
-                        if (a.rout) a.rout[ib] = u;
-                        if (a.rout2) a.rout2[ib] = rb;
-                    }
+                    if (va) pot_point<T>(a, ia, ra);
+                    if (vb) pot_point<T>(a, ib, rb);
                 }
             }
         }

@@ -27,7 +27,41 @@ template <typename T> struct XPassArgs {
     T cp4, cpp4, a0, a1;
     int lsolid;
     T csol, beta, den_m;
+    // fold: CG_S writes the local part of pot4, dencg^2 (a0 + a1 dencg)
+    // [+ penalty], instead of dencg, and POT reads it back as rin2 (no den)
+    int fold;
 };
+
+// Pointwise work of the x passes after an inverse transform, on element i of
+// a real line (shared by the Stockham and register passes).
+// CG_S: g = dencg; stores dencg (or the folded local part); returns
+// s = den dencg (cp4 + cpp4 dencg) (poten.f90:34-38, 62-64).
+template <typename T> __device__ __forceinline__ T cgs_point(const XPassArgs<T>& a, long i, T g) {
+    const T d = a.rin[i];
+    if (a.fold) {
+        T u = g * g * (a.a0 + a.a1 * g);
+        if (a.lsolid) u = u + solid_penalty<T>(d, a.csol, a.beta, a.den_m);
+        a.rout[i] = u;
+    } else {
+        a.rout[i] = g;
+    }
+    return d * g * (a.cp4 + a.cpp4 * g);
+}
+
+// POT: cv = c2r of the spectrum; pot4 = delj4 + dencg^2 (a0 + a1 dencg) +
+// sto1 [+ penalty] (poten.f90:80-94, 110-114), or cv + the folded local part
+template <typename T> __device__ __forceinline__ void pot_point(const XPassArgs<T>& a, long i, T cv) {
+    T u;
+    if (a.fold) {
+        u = a.rin2[i] + cv;
+    } else {
+        const T g = a.rin2[i];
+        u = (a.rin3 ? a.rin3[i] : (T)0) + g * g * (a.a0 + a.a1 * g) + cv;
+        if (a.lsolid) u = u + solid_penalty<T>(a.rin[i], a.csol, a.beta, a.den_m);
+    }
+    if (a.rout) a.rout[i] = u;
+    if (a.rout2) a.rout2[i] = cv;
+}
 
 template <typename T> struct YPassArgs {
     FftAxis ax;
