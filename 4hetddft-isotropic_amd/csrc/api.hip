@@ -150,7 +150,10 @@ template <typename T> static hipError_t run_x(hetd_ctx* c, XPassArgs<T> x) {
     case XMODE_R2C: b = s * n + 2 * s * nh; break;
     case XMODE_C2R: b = 2 * s * nh + s * n; break;
     case XMODE_CG_S: b = 4 * s * nh + 2 * s * n; break;
-    default: b = 2 * s * nh + 4 * s * n + (x.rout2 ? s * n : 0); break;
+    default:  // POT: spectrum, rin2 (dencg or the folded local part), [delj4], [den]; pot4, [sto1]
+        b = 2 * s * nh + s * n + (x.rin3 ? s * n : 0) + (!x.fold && x.lsolid ? s * n : 0) +
+            (x.rout ? s * n : 0) + (x.rout2 ? s * n : 0);
+        break;
     }
     tbeg(c, HETD_K_XPASS);
     hipError_t e = reg_fft_supported(x.ax.n) ? launch_xreg(x, c->st)
