@@ -1172,6 +1172,35 @@ int hetd_energy(hetd_ctx* c, hetd_energies* e) {
     return HETD_OK;
 }
 
+int hetd_moments(hetd_ctx* c, double npart, double out[10]) {
+    if (!c || !out || !(npart != 0.0)) return HETD_ERR_ARG;
+    if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
+    EnergyArgs a{};
+    a.g = c->grid; a.sc = c->sc; a.psi = c->psi; a.den = c->den; a.dencg = c->dencg; a.delj4 = c->delj4;
+    a.imp = c->imp; a.partial = c->partial;
+    double s[EN_NCOMP];
+    for (int pass = 1; pass <= 2; ++pass) {
+        a.mom = pass;
+        int nb = 0;
+        HCHK(c, tiled_supported(c->sc) ? launch_energy_tiled(a, c->st, &nb) : launch_energy_partial(a, c->st, &nb));
+        HCHK(c, launch_reduce_final(c->partial, nb, EN_NCOMP, c->red, c->st));
+        HCHK(c, hipMemcpyAsync(s, c->red, sizeof s, hipMemcpyDeviceToHost, c->st));
+        HCHK(c, hipStreamSynchronize(c->st));
+        if (pass == 1) {
+            // r_cm.f90:52-54 (divided by npart), DFT4He3d.f90:944-947, 869
+            for (int d = 0; d < 3; ++d) {
+                out[d] = s[d] * c->dxyz / npart;
+                out[3 + d] = s[3 + d] * c->dxyz;
+                a.cm[d] = out[d];
+            }
+            out[6] = s[6] * c->dxyz;
+        } else {
+            for (int d = 0; d < 3; ++d) out[7 + d] = s[d] * c->dxyz;  // DFT4He3d.f90:899,912,925
+        }
+    }
+    return HETD_OK;
+}
+
 int hetd_norm(hetd_ctx* c, double* norm) {
     if (!c || !norm) return HETD_ERR_ARG;
     EnergyArgs a{};

@@ -200,8 +200,35 @@ struct EnergyArgs {
     double* partial;  // EN_NCOMP per block
     const float* dencgf;  // mixed precision fields (else dencg / delj4)
     const float* delj4f;
+    int mom;          // 0: energy terms; 1: density moments; 2: angular momentum about cm
+    double cm[3];
 };
 hipError_t launch_energy_partial(const EnergyArgs& a, hipStream_t st, int* nblocks);
+
+// The diagnostics of the driver's energy block at one point (psi, its
+// gradient g1..g3, coordinates x, y, z), summed in place of the energy terms:
+//   mom 1: den x, den y, den z (r_cm.f90:38-50), den x^2, den y^2, den z^2
+//          (DFT4He3d.f90:935-947), den;
+//   mom 2: Re(i conj(psi) W) with W = (z-zcm) dpsi/dy - (y-ycm) dpsi/dz, and
+//          the two rotations of it: <Lx, Ly, Lz> (DFT4He3d.f90:883-925).
+__device__ __forceinline__ void moment_terms(const EnergyArgs& a, double* v, double2 ps, double2 g1, double2 g2,
+                                             double2 g3, double x, double y, double z) {
+    if (a.mom == 1) {
+        const double d = abs2(ps);
+        v[0] += d * x; v[1] += d * y; v[2] += d * z;
+        v[3] += d * (x * x); v[4] += d * (y * y); v[5] += d * (z * z);
+        v[6] += d;
+    } else {
+        const double xr = x - a.cm[0], yr = y - a.cm[1], zr = z - a.cm[2];
+        const double2 wx = make_double2(zr * g2.x - yr * g3.x, zr * g2.y - yr * g3.y);
+        const double2 wy = make_double2(xr * g3.x - zr * g1.x, xr * g3.y - zr * g1.y);
+        const double2 wz = make_double2(yr * g1.x - xr * g2.x, yr * g1.y - xr * g2.y);
+        // Re(i conj(psi) W) = -(psi.x W.y - psi.y W.x)
+        v[0] -= ps.x * wx.y - ps.y * wx.x;
+        v[1] -= ps.x * wy.y - ps.y * wy.x;
+        v[2] -= ps.x * wz.y - ps.y * wz.x;
+    }
+}
 
 // LDS-tiled versions (npd 9 and 13); one partial per tile
 bool tiled_supported(const StencilCoef& sc);

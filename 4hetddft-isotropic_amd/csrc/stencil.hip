@@ -237,17 +237,21 @@ __global__ void __launch_bounds__(TBX* TBY) k_energy(EnergyArgs a) {
         const double2 g2 = stencil_axis<double2>(a.psi, o - (long)g.nx * iy, iy, g.ny, g.nx, sc.c1[1], sc.npd, sc.m, sc.icon);
         const double2 g3 = stencil_axis<double2>(a.psi, o - (long)g.nx * g.ny * iz, iz, g.nz, (long)g.nx * g.ny,
                                                  sc.c1[2], sc.npd, sc.m, sc.icon);
-        v[EN_KIN] = (sq_abs(g1) + sq_abs(g2)) + sq_abs(g3);
-        const double d = a.den[o], cg = a.dencg[o];
-        v[EN_LJ] = a.delj4[o] * d;
-        v[EN_COR] = d * cg * cg * (a.aux1 + a.aux2 * cg);
-        if (a.lsolid) v[EN_SOLID] = d * (1.0 + tanh(a.beta * (d - a.den_m)));
-        const double* R = a.imp->rpot;
-        const double dz = g.z[iz] - R[2], dy = g.y[iy] - R[1];
-        v[EN_HEX] = uimp_at(g.potion, g.npot, g.delinter, g.x[ix] - R[0], dy * dy + dz * dz) * d;
-        if (a.otc)
-            v[EN_ALPHA] = a.falfs[o] * ((a.dxden[o] * a.intx[o] + a.dyden[o] * a.inty[o]) + a.dzden[o] * a.intz[o]);
-        v[EN_NORM] = d;
+        if (a.mom) {
+            moment_terms(a, v, a.psi[o], g1, g2, g3, g.x[ix], g.y[iy], g.z[iz]);
+        } else {
+            v[EN_KIN] = (sq_abs(g1) + sq_abs(g2)) + sq_abs(g3);
+            const double d = a.den[o], cg = a.dencg[o];
+            v[EN_LJ] = a.delj4[o] * d;
+            v[EN_COR] = d * cg * cg * (a.aux1 + a.aux2 * cg);
+            if (a.lsolid) v[EN_SOLID] = d * (1.0 + tanh(a.beta * (d - a.den_m)));
+            const double* R = a.imp->rpot;
+            const double dz = g.z[iz] - R[2], dy = g.y[iy] - R[1];
+            v[EN_HEX] = uimp_at(g.potion, g.npot, g.delinter, g.x[ix] - R[0], dy * dy + dz * dz) * d;
+            if (a.otc)
+                v[EN_ALPHA] = a.falfs[o] * ((a.dxden[o] * a.intx[o] + a.dyden[o] * a.inty[o]) + a.dzden[o] * a.intz[o]);
+            v[EN_NORM] = d;
+        }
     }
     __shared__ double red[EN_NCOMP * TBX * TBY / 64];
     block_sum<EN_NCOMP>(v, red);

@@ -449,6 +449,10 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                 acc[2] += (double)sz * u;
             } else {
                 const EnergyArgs& a = args;
+                if (a.mom) {
+                    moment_terms(a, acc, q[p][M], sx, sy, sz, g.x[ix], g.y[iy[p]], g.z[iz]);
+                    continue;
+                }
                 acc[EN_KIN] += (abs2(sx) + abs2(sy)) + abs2(sz);
                 // den = |psi|^2 of the FP64 state (equal to the den array in FP64)
                 const double d = abs2(q[p][M]);

@@ -115,6 +115,12 @@ module hetd_capi
       integer(c_int32_t), value :: nsteps
       real(c_double), dimension(*), intent(out) :: traj
     end function
+    integer(c_int) function hetd_moments(ctx, npart, out) bind(C, name='hetd_moments')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: npart
+      real(c_double), dimension(10), intent(out) :: out
+    end function
     integer(c_int) function hetd_pool_evolve(ctxs, n, dt, nsteps) bind(C, name='hetd_pool_evolve')
       import :: c_int, c_int32_t, c_double, c_ptr
       type(c_ptr), dimension(*), intent(in) :: ctxs

@@ -71,11 +71,10 @@ program hetd_drive
   real(8), parameter :: ktops = 7.63822291d0, pstok = 1.d0/7.63822291d0, mp_u = 0.02061484d0
   type(hetd_params) :: p
   complex(8), allocatable, target :: psi(:,:,:)
-  real(8), allocatable, target :: den(:,:,:)
   real(8), allocatable :: traj(:,:)
   real(8) :: rimp(3), vimp(3), aimp(3), mAg_u, auxn4, eold, temps
   real(8) :: hx, hy, hz, t_start, t_end
-  real(8) :: cm(3), r2(3)
+  real(8) :: cm(3), r2(3), ang(3)
   integer :: iter, iteraux, nchunk, s, next_out, ncurr
   integer(c_int) :: rc
   integer(8) :: c1, c2, crate
@@ -121,7 +120,7 @@ program hetd_drive
   call fftini(nx, ny, nz, nthread)
 
   ! readenc mode 2 (readenc.f90:104-113)
-  allocate(psi(nx,ny,nz), den(nx,ny,nz))
+  allocate(psi(nx,ny,nz))
   call read_mode2(filedenin)
   rc = hetd_set_state(hctx, psi, rimp, vimp)
   call hetd_check(rc, 'hetd_set_state')
@@ -195,6 +194,7 @@ program hetd_drive
       write(6, 7015) eimpu, ekinx, eHeX, 0.d0, etot
       eold = etot4
       call moments()
+      write(6, '("<Lx,Ly,Lz>.......:",1p,3E20.11)') ang
       write(6, '(/1x,T5,"Center of Mass of the Helium ...(",0P,F9.4,",",F9.4,",",F9.4,") A")') cm
       write(6, '(" Impurity position:",1p,3E15.6)') rimp
       write(6, '(" Impurity velocity:",1p,3E15.6)') vimp
@@ -282,27 +282,13 @@ contains
   end subroutine read_mode2
 
   subroutine moments()
-    ! centre of mass and <x2,y2,z2> from den (r_cm.f90:26-50, DFT4He3d.f90:935-947)
-    integer :: ix, iy, iz
-    real(8) :: xx, yy, zz, dxyz
-    call hetd_check(hetd_get_field(hctx, HETD_F_DEN, c_loc(den)), 'get_field')
-    dxyz = hx*hy*hz
-    cm = 0; r2 = 0
-    do iz = 1, nz
-      zz = -zmax + hz*(iz-1)
-      do iy = 1, ny
-        yy = -ymax + hy*(iy-1)
-        do ix = 1, nx
-          xx = -xmax + hx*(ix-1)
-          cm(1) = cm(1) + den(ix,iy,iz)*xx; cm(2) = cm(2) + den(ix,iy,iz)*yy
-          cm(3) = cm(3) + den(ix,iy,iz)*zz
-          r2(1) = r2(1) + den(ix,iy,iz)*xx**2; r2(2) = r2(2) + den(ix,iy,iz)*yy**2
-          r2(3) = r2(3) + den(ix,iy,iz)*zz**2
-        end do
-      end do
-    end do
-    cm = cm*dxyz/n4
-    r2 = r2*dxyz
+    ! centre of mass (r_cm.f90:23-55), <Lx,Ly,Lz> about it (DFT4He3d.f90:883-925)
+    ! and <x2,y2,z2> (DFT4He3d.f90:935-947), reduced on the device
+    real(8) :: mom(10)
+    call hetd_check(hetd_moments(hctx, dble(n4), mom), 'moments')
+    cm = mom(1:3)
+    r2 = mom(4:6)
+    ang = mom(8:10)
   end subroutine moments
 
   subroutine write_mode2(fname, it, ttime)
@@ -333,7 +319,7 @@ contains
     write(10, '("#  Ekinx......:",1p,E16.8)') ekinx
     write(10, '("#  Eimpu......:",1p,E16.8)') eimpu
     write(10, '("#  Etot.......:",1p,E16.8)') etot
-    write(10, '("#  <Lx,y,z>...:",1p,3E16.8)') 0.d0, 0.d0, 0.d0
+    write(10, '("#  <Lx,y,z>...:",1p,3E16.8)') ang
     write(10, '("#  <x2,y2,z2>.:",1p,3E16.8)') r2
     write(10, '("#  ")')
     write(10, *) xmax, ymax, zmax, hx, hy, hz, nx, ny, nz

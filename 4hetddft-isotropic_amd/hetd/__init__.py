@@ -130,6 +130,12 @@ class Context:
         self._chk(self.L.hetd_norm(self.h, C.byref(v)))
         return v.value
 
+    def moments(self, npart):
+        """dict: cm (3), r2 (3), n, L (3) -- the energy block's diagnostics."""
+        out = np.zeros(10)
+        self._chk(self.L.hetd_moments(self.h, npart, _p(out)))
+        return {"cm": out[0:3].copy(), "r2": out[3:6].copy(), "n": out[6], "L": out[7:10].copy()}
+
     def evolve(self, dt, nsteps, traj=True):
         t = np.zeros((max(nsteps, 1), capi.TRAJ_STRIDE))
         self._chk(self.L.hetd_evolve(self.h, dt, nsteps, _p(t) if traj else None))

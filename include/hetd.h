@@ -140,6 +140,13 @@ int hetd_steppc(hetd_ctx *ctx, double deltat, double *errHe, double *errimp);
 int hetd_energy(hetd_ctx *ctx, hetd_energies *e);
 /* sum(den)*dxyz (DFT4He3d.f90:869). */
 int hetd_norm(hetd_ctx *ctx, double *norm);
+/* The diagnostics of the driver's energy block (DFT4He3d.f90:868-947), as
+ * device reductions: out[0..2] = centre of mass sum(den*x)*dxyz/npart
+ * (r_cm.f90:23-55), out[3..5] = sum(den*x^2)*dxyz (not about the centre),
+ * out[6] = sum(den)*dxyz, out[7..9] = <Lx,Ly,Lz> =
+ * Re(sum(i conj(psi) ((y-ycm) d/dz ... )))*dxyz about that centre of mass,
+ * with the first-derivative stencil of derivnD. */
+int hetd_moments(hetd_ctx *ctx, double npart, double out[10]);
 /* nsteps iterations of the driver loop body (DFT4He3d.f90:840-852): steprk
  * for the first 3 iterations after hetd_set_state, steppc afterwards, then
  * potenimp/poten/forceimp and aimp = F/mAg_u at the new state -- entirely on

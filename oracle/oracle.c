@@ -903,6 +903,65 @@ int orc_fft_c2r(int nx, int ny, int nz, const double *in, double *out) {
     return 0;
 }
 
+/* The energy block's diagnostics (DFT4He3d.f90:868-947): r_cm (r_cm.f90:23-55),
+ * <Lx,Ly,Lz> about it with complex sums as written, <x2,y2,z2> and the norm. */
+int orc_moments(orc_ctx *c, double npart, double out[10]) {
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    double xcm = 0.0, ycm = 0.0, zcm = 0.0;
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                const double aux = c->den[ix + (size_t)nx * (iy + (size_t)ny * iz)];
+                xcm = xcm + aux * c->x[ix];
+                ycm = ycm + aux * c->y[iy];
+                zcm = zcm + aux * c->z[iz];
+            }
+    xcm = xcm * c->dxyz / npart;
+    ycm = ycm * c->dxyz / npart;
+    zcm = zcm * c->dxyz / npart;
+    deriv_field(c, 1, 1, 1, c->psi, c->s1c);
+    deriv_field(c, 1, 2, 1, c->psi, c->s2c);
+    deriv_field(c, 1, 3, 1, c->psi, c->s3c);
+    /* caux = sum i conj(psi) W, W per component as DFT4He3d.f90:894-895,907-908,920-921 */
+    double cre[3] = {0, 0, 0}, cim[3] = {0, 0, 0};
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                const size_t i = ix + (size_t)nx * (iy + (size_t)ny * iz);
+                const double xr = c->x[ix] - xcm, yr = c->y[iy] - ycm, zr = c->z[iz] - zcm;
+                const double pr = c->psi[2 * i], pim = -c->psi[2 * i + 1]; /* conj(psi) */
+                const double d1r = c->s1c[2 * i], d1i = c->s1c[2 * i + 1];
+                const double d2r = c->s2c[2 * i], d2i = c->s2c[2 * i + 1];
+                const double d3r = c->s3c[2 * i], d3i = c->s3c[2 * i + 1];
+                double wr[3], wi[3];
+                wr[2] = yr * d1r - xr * d2r; wi[2] = yr * d1i - xr * d2i; /* z */
+                wr[1] = xr * d3r - zr * d1r; wi[1] = xr * d3i - zr * d1i; /* y */
+                wr[0] = zr * d2r - yr * d3r; wi[0] = zr * d2i - yr * d3i; /* x */
+                for (int k = 0; k < 3; ++k) {
+                    /* i * conj(psi) * W */
+                    const double tr = pr * wr[k] - pim * wi[k], ti = pr * wi[k] + pim * wr[k];
+                    cre[k] += -ti;
+                    cim[k] += tr;
+                }
+            }
+    double a1 = 0.0, a2 = 0.0, a3 = 0.0, s = 0.0;
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                const double d = c->den[ix + (size_t)nx * (iy + (size_t)ny * iz)];
+                a1 = a1 + d * c->x[ix] * c->x[ix];
+                a2 = a2 + d * c->y[iy] * c->y[iy];
+                a3 = a3 + d * c->z[iz] * c->z[iz];
+                s += d;
+            }
+    (void)cim;
+    out[0] = xcm; out[1] = ycm; out[2] = zcm;
+    out[3] = a1 * c->dxyz; out[4] = a2 * c->dxyz; out[5] = a3 * c->dxyz;
+    out[6] = s * c->dxyz;
+    for (int k = 0; k < 3; ++k) out[7 + k] = cre[k] * c->dxyz; /* real(caux)*dxyz */
+    return 0;
+}
+
 int orc_deriv(int nx, int ny, int nz, int npd, int icon, int id, int iv, double h,
               int is_complex, const double *f, double *df) {
     return deriv3d(nx, ny, nz, npd, icon, id, iv, h, is_complex, f, df);

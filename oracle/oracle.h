@@ -37,6 +37,8 @@ int orc_steprk(orc_ctx *c, double dt);
 int orc_steppc(orc_ctx *c, double dt, double *errHe, double *errimp);
 int orc_energy(orc_ctx *c, hetd_energies *e);
 int orc_norm(orc_ctx *c, double *norm);
+/* DFT4He3d.f90:868-947: cm (r_cm.f90), <x2,y2,z2>, norm, <Lx,Ly,Lz> (out[0..9]) */
+int orc_moments(orc_ctx *c, double npart, double out[10]);
 /* one iteration of the driver loop body (DFT4He3d.f90:840-852) */
 int orc_evolve(orc_ctx *c, double dt, int nsteps, double *traj /* nsteps*HETD_TRAJ_STRIDE or NULL */);
 

@@ -51,6 +51,7 @@ def lib():
         L.orc_steppc.argtypes = [_vp, C.c_double, _dp, _dp]
         L.orc_energy.argtypes = [_vp, C.POINTER(capi.Energies)]
         L.orc_norm.argtypes = [_vp, _dp]
+        L.orc_moments.argtypes = [_vp, C.c_double, _dp]
         L.orc_evolve.argtypes = [_vp, C.c_double, C.c_int, _dp]
         L.orc_get_field.argtypes = [_vp, C.c_int, _dp]
         L.orc_get_imp.argtypes = [_vp, _dp, _dp, _dp]
@@ -157,6 +158,11 @@ class OracleCtx:
         v = C.c_double()
         self._chk(self.L.orc_norm(self.h, C.byref(v)))
         return v.value
+
+    def moments(self, npart):
+        out = np.zeros(10)
+        self._chk(self.L.orc_moments(self.h, npart, _p(out)))
+        return {"cm": out[0:3].copy(), "r2": out[3:6].copy(), "n": out[6], "L": out[7:10].copy()}
 
     def evolve(self, dt, nsteps):
         t = np.zeros((nsteps, capi.TRAJ_STRIDE))

@@ -184,6 +184,36 @@ def test_evolve_matches_oracle_other_grids(oracle_mod, shape, npd, icon, core):
         assert abs(e1[k] - e2[k]) <= TOL * max(abs(e2[k]), 1.0), k
 
 
+@pytest.mark.parametrize("npd,icon,prec", [(13, 13, capi.HETD_FP64), (9, 8, capi.HETD_FP64), (7, 13, capi.HETD_FP64),
+                                           (13, 13, capi.HETD_MIXED)])
+def test_moments_match_oracle(oracle_mod, npd, icon, prec):
+    """Centre of mass, <x2,y2,z2>, norm and <Lx,Ly,Lz> of the energy block
+    (hetd_moments) against the oracle, on a droplet given some circulation."""
+    say("moments", npd, icon, prec)
+    from hetd.initstate import woods_saxon_state
+    nz, ny, nx = 36, 40, 48
+    h = 0.6
+    kw = dict(nx=nx, ny=ny, nz=nz, xmax=nx * h / 2, ymax=ny * h / 2, zmax=nz * h / 2, npd=npd, icon=icon)
+    psi = woods_saxon_state(nx, ny, nz, kw["xmax"], kw["ymax"], kw["zmax"], 150, 3, center=(0.7, -0.4, 0.3))
+    x = -kw["xmax"] + h * np.arange(nx)
+    y = -kw["ymax"] + h * np.arange(ny)
+    z = -kw["zmax"] + h * np.arange(nz)
+    zz, yy, xx = np.meshgrid(z, y, x, indexing="ij")
+    psi = psi * (1.0 + 0.2 * ((xx - 0.7) + 1j * (yy + 0.4)) + 0.1j * (zz - 0.3))
+    orc = oracle_mod.OracleCtx(capi.default_params(**kw))
+    orc.set_state(psi, (0, 0, 0), (0, 0, 0))
+    want = orc.moments(150.0)
+    with Context(capi.default_params(precision=prec, **kw)) as ctx:
+        ctx.set_state(psi, (0, 0, 0), (0, 0, 0))
+        got = ctx.moments(150.0)
+    orc.close()
+    for k in ("cm", "r2"):
+        assert np.abs(got[k] - want[k]).max() <= TOL * np.abs(want[k]).max(), k
+    assert abs(got["n"] - want["n"]) <= TOL * want["n"]
+    assert np.abs(want["L"]).max() > 1.0  # the state does carry angular momentum
+    assert np.abs(got["L"] - want["L"]).max() <= 1e-9 * np.abs(want["L"]).max()
+
+
 def test_guard_reports_nonpositive_density():
     say("guard")
     p = capi.default_params(nx=16, ny=16, nz=16, xmax=4.0, ymax=4.0, zmax=4.0, npd=9)

@@ -262,6 +262,40 @@ def test_force_vanishes_for_symmetric_density(oracle_mod):
     assert abs(F[0]) > 1.0  # and the probe is sensitive
 
 
+def test_moments_closed_forms(oracle_mod):
+    """Centre of mass, <x^2> and <L> of the energy block (DFT4He3d.f90:868-947)
+    against closed forms: a Gaussian on a grid symmetric about its centre,
+    and an m = 1 vortex, whose <Lz> is the particle number (Lx = Ly = 0)."""
+    n, h, s = 48, 0.3, 1.2
+    xm = n * h / 2
+    p = capi.default_params(nx=n, ny=n, nz=n, xmax=xm, ymax=xm, zmax=xm, npd=13, icon=13)
+    x = -xm + h * np.arange(n)
+    c = -h / 2  # the grid is symmetric about -h/2
+    zz, yy, xx = np.meshgrid(x - c, x - c, x - c, indexing="ij")
+    g = np.exp(-(xx ** 2 + yy ** 2 + zz ** 2) / (2 * s * s))
+    ctx = oracle_mod.OracleCtx(p)
+    # Gaussian: cm = c, sum(den x^2) dV = pi^(3/2) s^3 (s^2/2 + c^2)
+    ctx.set_state(g + 0j, (0, 0, 0), (0, 0, 0))
+    m = ctx.moments(1.0)
+    nrm = np.pi ** 1.5 * s ** 3
+    assert abs(m["n"] - nrm) < 1e-12 * nrm
+    m = ctx.moments(m["n"])
+    assert np.abs(m["cm"] - c).max() < 1e-13
+    assert np.abs(m["r2"] - nrm * (s * s / 2 + c * c)).max() < 1e-11 * nrm
+    assert np.abs(m["L"]).max() < 1e-12 * nrm  # real wavefunction: no current
+    # vortex (x + i y) g about the centre: Lz = N, Lx = Ly = 0 (FD error ~ h^12)
+    ctx.set_state((xx + 1j * yy) * g, (0, 0, 0), (0, 0, 0))
+    N = ctx.moments(1.0)["n"]
+    m = ctx.moments(N)
+    assert abs(m["L"][2] - N) < 1e-7 * N
+    assert np.abs(m["L"][:2]).max() < 1e-12 * N
+    assert np.abs(m["cm"] - c).max() < 1e-13
+    # the opposite circulation flips the sign
+    ctx.set_state((xx - 1j * yy) * g, (0, 0, 0), (0, 0, 0))
+    assert abs(ctx.moments(N)["L"][2] + N) < 1e-7 * N
+    ctx.close()
+
+
 def _scalar_rk_pc(fun, dt, nsteps, y0):
     """Scalar recurrences of RK-Gill (steprk.f90:32-92) for the first three
     steps and Hamming predictor-modifier-corrector (steppc.f90:45-134) after,
