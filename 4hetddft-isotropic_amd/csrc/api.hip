@@ -1026,6 +1026,90 @@ int hetd_set_state(hetd_ctx* c, const double* psi, const double r[3], const doub
     return HETD_OK;
 }
 
+// ----------------------------------------------------- integrator history
+//
+// Blob: header, the device ImpState, then psiold[0..2], hpsiold[0..1]
+// (complex64 in mixed precision), pc and q, each nx*ny*nz values.  With the
+// current psi of a mode-2 file it restarts a run bit for bit (the reference
+// keeps no history and restarts with Runge-Kutta steps).
+
+namespace {
+struct HistHeader {
+    char magic[8];
+    int32_t version, nx, ny, nz, precision;
+    int32_t ioldp[3], ioldh[2], ioldr[3], ioldv[3], iolda[2];
+    int32_t pad;
+    int64_t iter_since_set, bytes;
+};
+constexpr char kHistMagic[8] = {'H', 'E', 'T', 'D', 'H', 'I', 'S', 'T'};
+}  // namespace
+
+static size_t history_bytes(const hetd_ctx* c) {
+    const size_t nC = sizeof(double2) * c->n, nHist = (c->mixed ? sizeof(float2) : sizeof(double2)) * c->n;
+    return sizeof(HistHeader) + sizeof(ImpState) + 5 * nC + 2 * nHist;
+}
+
+int64_t hetd_history_size(hetd_ctx* c) { return c ? (int64_t)history_bytes(c) : -1; }
+
+int hetd_save_history(hetd_ctx* c, void* host) {
+    if (!c || !host) return HETD_ERR_ARG;
+    if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
+    if (int rc = settle_force(c)) return rc;
+    HistHeader h{};
+    std::memcpy(h.magic, kHistMagic, 8);
+    h.version = 1; h.nx = c->nx; h.ny = c->ny; h.nz = c->nz; h.precision = c->mixed ? HETD_MIXED : HETD_FP64;
+    for (int i = 0; i < 3; ++i) { h.ioldp[i] = c->ioldp[i]; h.ioldr[i] = c->ioldr[i]; h.ioldv[i] = c->ioldv[i]; }
+    for (int i = 0; i < 2; ++i) { h.ioldh[i] = c->ioldh[i]; h.iolda[i] = c->iolda[i]; }
+    h.iter_since_set = c->iter_since_set;
+    h.bytes = (int64_t)history_bytes(c);
+    char* p = static_cast<char*>(host);
+    std::memcpy(p, &h, sizeof h);
+    p += sizeof h;
+    const size_t nC = sizeof(double2) * c->n, nHist = (c->mixed ? sizeof(float2) : sizeof(double2)) * c->n;
+    HCHK(c, hipMemcpyAsync(p, c->imp, sizeof(ImpState), hipMemcpyDeviceToHost, c->st));
+    p += sizeof(ImpState);
+    for (int i = 0; i < 3; ++i, p += nC) HCHK(c, hipMemcpyAsync(p, c->psiold[i], nC, hipMemcpyDeviceToHost, c->st));
+    for (int i = 0; i < 2; ++i, p += nHist)
+        HCHK(c, hipMemcpyAsync(p, c->hpsiold[i], nHist, hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipMemcpyAsync(p, c->pc, nC, hipMemcpyDeviceToHost, c->st));
+    p += nC;
+    HCHK(c, hipMemcpyAsync(p, c->q, nC, hipMemcpyDeviceToHost, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    return HETD_OK;
+}
+
+int hetd_load_history(hetd_ctx* c, const void* host, int64_t bytes) {
+    if (!c || !host) return HETD_ERR_ARG;
+    if (!c->have_state) return fail(c, HETD_ERR_STATE, "hetd_set_state must precede hetd_load_history");
+    HistHeader h;
+    if (bytes < (int64_t)sizeof h) return fail(c, HETD_ERR_ARG, "history blob too short");
+    std::memcpy(&h, host, sizeof h);
+    if (std::memcmp(h.magic, kHistMagic, 8) != 0 || h.version != 1)
+        return fail(c, HETD_ERR_ARG, "not a history blob of this library");
+    if (h.nx != c->nx || h.ny != c->ny || h.nz != c->nz || h.precision != (c->mixed ? HETD_MIXED : HETD_FP64) ||
+        h.bytes != (int64_t)history_bytes(c) || bytes < h.bytes)
+        return fail(c, HETD_ERR_ARG, "history blob is for another grid or precision");
+    drop_graph(c);
+    const char* p = static_cast<const char*>(host) + sizeof h;
+    const size_t nC = sizeof(double2) * c->n, nHist = (c->mixed ? sizeof(float2) : sizeof(double2)) * c->n;
+    HCHK(c, hipMemcpyAsync(c->imp, p, sizeof(ImpState), hipMemcpyHostToDevice, c->st));
+    p += sizeof(ImpState);
+    for (int i = 0; i < 3; ++i, p += nC) HCHK(c, hipMemcpyAsync(c->psiold[i], p, nC, hipMemcpyHostToDevice, c->st));
+    for (int i = 0; i < 2; ++i, p += nHist)
+        HCHK(c, hipMemcpyAsync(c->hpsiold[i], p, nHist, hipMemcpyHostToDevice, c->st));
+    HCHK(c, hipMemcpyAsync(c->pc, p, nC, hipMemcpyHostToDevice, c->st));
+    p += nC;
+    HCHK(c, hipMemcpyAsync(c->q, p, nC, hipMemcpyHostToDevice, c->st));
+    for (int i = 0; i < 3; ++i) { c->ioldp[i] = h.ioldp[i]; c->ioldr[i] = h.ioldr[i]; c->ioldv[i] = h.ioldv[i]; }
+    for (int i = 0; i < 2; ++i) { c->ioldh[i] = h.ioldh[i]; c->iolda[i] = h.iolda[i]; }
+    c->iter_since_set = h.iter_since_set;
+    c->force_pending = false;  // the saved acceleration is that of the saved state
+    // the mean field of the restored state, as the last step left it
+    if (int rc = poten_dev(c)) return rc;
+    HCHK(c, hipStreamSynchronize(c->st));
+    return HETD_OK;
+}
+
 int hetd_potenimp(hetd_ctx* c, const double r[3]) {
     if (!c || !r) return HETD_ERR_ARG;
     HCHK(c, hipMemcpyAsync(c->imp->rpot, r, 3 * sizeof(double), hipMemcpyHostToDevice, c->st));

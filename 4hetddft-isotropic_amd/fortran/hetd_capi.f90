@@ -115,6 +115,19 @@ module hetd_capi
       integer(c_int32_t), value :: nsteps
       real(c_double), dimension(*), intent(out) :: traj
     end function
+    integer(c_int64_t) function hetd_history_size(ctx) bind(C, name='hetd_history_size')
+      import :: c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hetd_save_history(ctx, host) bind(C, name='hetd_save_history')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, host
+    end function
+    integer(c_int) function hetd_load_history(ctx, host, bytes) bind(C, name='hetd_load_history')
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx, host
+      integer(c_int64_t), value :: bytes
+    end function
     integer(c_int) function hetd_moments(ctx, npart, out) bind(C, name='hetd_moments')
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx

@@ -46,6 +46,9 @@ program hetd_drive
   real(8) :: vximp = 0, vyimp = 0, vzimp = 0
   character(len=80) :: selec = '', selec_gs = ''
   real(8) :: r_cutoff = 2.0d0, umax = 400.d0, r_cutoff_gs = -1.d0, umax_gs = -1.d0
+  ! extension (not a reference key): keep the integrator history beside every
+  ! checkpoint (<file>.hist) and restart from <filedenin>.hist when present
+  logical :: lhistory = .false.
 
   namelist /input/title,fftwplan,nthread,nsfiles,                         &
                   fileout,filemonopol, filerimp,filevimp,fileaimp,        &
@@ -66,7 +69,7 @@ program hetd_drive
                   pcurr,icurr,lsurf,lsurf3D,Lambdah,                      &
                   txmean,txsurf,tymean,tysurf,tzmean,tzsurf,              &
                   vximp,vyimp,vzimp,selec,r_cutoff,umax,                  &
-                  selec_gs,r_cutoff_gs,umax_gs
+                  selec_gs,r_cutoff_gs,umax_gs,lhistory
 
   real(8), parameter :: ktops = 7.63822291d0, pstok = 1.d0/7.63822291d0, mp_u = 0.02061484d0
   type(hetd_params) :: p
@@ -138,6 +141,7 @@ program hetd_drive
   write(6, 6050) etot4, etot4/n4, ekin4, elj4, ealphas, esolid, ecor4
   write(6, 6060) eimpu, ekinx, eHeX, 0.d0, etot
   eold = etot4
+  if (lhistory) call read_history(trim(filedenin)//'.hist')
 
   open(81, file=filerimp); open(82, file=filevimp); open(83, file=fileaimp)
   write(81, '("# Tiempo(ps), x(AA), y(AA), z(AA)")')
@@ -327,6 +331,36 @@ contains
     write(10, *) vimp
     write(10, '(2("(",ES24.16E3,",",ES24.16E3,")",1x))') psi
     close(10)
+    if (lhistory) call write_history(trim(fname)//'.hist')
   end subroutine write_mode2
+
+  subroutine write_history(fname)
+    character(len=*), intent(in) :: fname
+    integer(c_int8_t), allocatable, target :: blob(:)
+    allocate(blob(hetd_history_size(hctx)))
+    call hetd_check(hetd_save_history(hctx, c_loc(blob)), 'save_history')
+    open(11, file=fname, access='stream', form='unformatted', status='replace')
+    write(11) blob
+    close(11)
+  end subroutine write_history
+
+  subroutine read_history(fname)
+    character(len=*), intent(in) :: fname
+    integer(c_int8_t), allocatable, target :: blob(:)
+    integer(8) :: nbytes
+    logical :: there
+    inquire(file=fname, exist=there, size=nbytes)
+    if (.not. there) then
+      write(6, '(" No integrator history in ",A,": the run starts with Runge-Kutta steps")') fname
+      return
+    end if
+    allocate(blob(nbytes))
+    open(11, file=fname, access='stream', form='unformatted', status='old')
+    read(11) blob
+    close(11)
+    call hetd_check(hetd_load_history(hctx, c_loc(blob), int(nbytes, c_int64_t)), 'load_history')
+    call hetd_check(hetd_get_imp(hctx, rimp, vimp, aimp), 'get_imp')
+    write(6, '(" Integrator history restored from ",A)') fname
+  end subroutine read_history
 
 end program hetd_drive

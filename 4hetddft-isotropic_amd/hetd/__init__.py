@@ -136,6 +136,18 @@ class Context:
         self._chk(self.L.hetd_moments(self.h, npart, _p(out)))
         return {"cm": out[0:3].copy(), "r2": out[3:6].copy(), "n": out[6], "L": out[7:10].copy()}
 
+    def save_history(self):
+        """The integrator history as bytes (hetd_save_history)."""
+        n = self.L.hetd_history_size(self.h)
+        buf = C.create_string_buffer(n)
+        self._chk(self.L.hetd_save_history(self.h, buf))
+        return buf.raw
+
+    def load_history(self, blob):
+        """Restore a history saved by save_history (after set_state)."""
+        buf = C.create_string_buffer(blob, len(blob))
+        self._chk(self.L.hetd_load_history(self.h, buf, len(blob)))
+
     def evolve(self, dt, nsteps, traj=True):
         t = np.zeros((max(nsteps, 1), capi.TRAJ_STRIDE))
         self._chk(self.L.hetd_evolve(self.h, dt, nsteps, _p(t) if traj else None))

@@ -116,6 +116,10 @@ def _proto(lib):
     lib.hetd_energy.argtypes = [_vp, C.POINTER(Energies)]
     lib.hetd_norm.argtypes = [_vp, _dp]
     lib.hetd_moments.argtypes = [_vp, C.c_double, _dp]
+    lib.hetd_history_size.argtypes = [_vp]
+    lib.hetd_history_size.restype = C.c_int64
+    lib.hetd_save_history.argtypes = [_vp, _vp]
+    lib.hetd_load_history.argtypes = [_vp, _vp, C.c_int64]
     lib.hetd_evolve.argtypes = [_vp, C.c_double, C.c_int32, _dp]
     lib.hetd_pool_evolve.argtypes = [C.POINTER(_vp), C.c_int32, C.c_double, C.c_int32]
     lib.hetd_get_field.argtypes = [_vp, C.c_int32, _dp]
@@ -132,7 +136,8 @@ def _proto(lib):
     for name in ("hetd_init", "hetd_finalize", "hetd_vimp", "hetd_build_potion",
                  "hetd_set_potion", "hetd_set_state", "hetd_potenimp", "hetd_poten",
                  "hetd_forceimp", "hetd_set_aimp", "hetd_steprk", "hetd_steppc",
-                 "hetd_energy", "hetd_norm", "hetd_moments", "hetd_evolve", "hetd_pool_evolve", "hetd_get_field",
+                 "hetd_energy", "hetd_norm", "hetd_moments", "hetd_save_history", "hetd_load_history",
+                 "hetd_evolve", "hetd_pool_evolve", "hetd_get_field",
                  "hetd_get_imp", "hetd_fft_r2c", "hetd_fft_c2r", "hetd_deriv"):
         getattr(lib, name).restype = C.c_int
     return lib
@@ -142,7 +147,7 @@ EXPORTED = ("hetd_default_params", "hetd_init", "hetd_finalize", "hetd_last_erro
             "hetd_vimp", "hetd_build_potion", "hetd_set_potion", "hetd_set_state",
             "hetd_potenimp", "hetd_poten", "hetd_forceimp", "hetd_set_aimp",
             "hetd_steprk", "hetd_steppc", "hetd_energy", "hetd_norm", "hetd_moments", "hetd_evolve",
-            "hetd_pool_evolve",
+            "hetd_history_size", "hetd_save_history", "hetd_load_history", "hetd_pool_evolve",
             "hetd_get_field", "hetd_get_imp", "hetd_fft_r2c", "hetd_fft_c2r",
             "hetd_deriv", "hetd_stream", "hetd_timing", "hetd_timing_read")
 

@@ -154,6 +154,19 @@ int hetd_moments(hetd_ctx *ctx, double npart, double out[10]);
  * receives nsteps*HETD_TRAJ_STRIDE doubles. */
 int hetd_evolve(hetd_ctx *ctx, double deltat, int32_t nsteps, double *traj);
 
+/* Integrator history for a bit-identical restart (no reference counterpart:
+ * the reference's checkpoint holds psi, R and v only, and a restart begins
+ * again with Runge-Kutta steps, DFT4He3d.f90:840).  The blob holds the ring
+ * of psiold/hpsiold, pc, q, the ring indices, the step count and the
+ * impurity state with its histories; hetd_history_size gives its size in
+ * bytes.  hetd_save_history forms a pending impurity force first.
+ * hetd_load_history follows hetd_set_state with the saved psi, R, v and
+ * recomputes the mean field; the run then continues as the saved one would
+ * have after its last hetd_evolve. */
+int64_t hetd_history_size(hetd_ctx *ctx);
+int hetd_save_history(hetd_ctx *ctx, void *host);
+int hetd_load_history(hetd_ctx *ctx, const void *host, int64_t bytes);
+
 /* Ensemble of independent runs (no reference counterpart: the reference
  * runs one droplet per process).  Advances each of the n contexts by nsteps
  * iterations exactly as hetd_evolve(ctx[i], deltat, nsteps, NULL) would --

@@ -64,3 +64,32 @@ def test_driver_reproduces_reference_run(tmp_path):
     psi, grid, r_, v_ = read_mode2(os.path.join(tmp_path, "den.out"))
     assert G.rel(G.planes(psi), d["final_planes"]) < 1e-10
     assert "elapsed partie iterative" in r.stdout
+
+
+@pytest.mark.gpu
+def test_driver_restart_with_history_is_bitwise(tmp_path):
+    """lhistory (an extension key): a run restarted from a checkpoint and its
+    integrator history continues bit for bit as the uninterrupted run."""
+    cfg, d = G.load("ot_xe_32")
+    full, part = os.path.join(tmp_path, "full"), os.path.join(tmp_path, "part")
+    for w in (full, part):
+        os.makedirs(w)
+        write_mode2(os.path.join(w, "den.in"), d["psi0"], cfg["xmax"], cfg["ymax"], cfg["zmax"],
+                    d["rimp0"], d["vimp0"])
+    r = run_drive(full, namelist(cfg, niter=16, pcurr=8, lhistory=".true."))
+    assert r.returncode == 0, r.stderr
+    ck = [f for f in sorted(os.listdir(full)) if f.endswith(".hist") and not f.startswith("den.out")]
+    assert ck, os.listdir(full)
+    first = ck[0][:-5]  # the checkpoint after 8 iterations
+    os.replace(os.path.join(full, first), os.path.join(part, "den.ck"))
+    os.replace(os.path.join(full, ck[0]), os.path.join(part, "den.ck.hist"))
+    r2 = run_drive(part, namelist(cfg, niter=16, pcurr=1000000, lhistory=".true.", filedenin="'den.ck'",
+                                  iter0=8, time0=repr(8 * 5e-4)))
+    assert r2.returncode == 0, r2.stderr
+    assert "Integrator history restored" in r2.stdout
+    a = np.loadtxt(os.path.join(full, "rimp.out"), comments="#")
+    b = np.loadtxt(os.path.join(part, "rimp.out"), comments="#")
+    assert np.array_equal(a[-8:, 1:], b[-8:, 1:])
+    psa = read_mode2(os.path.join(full, "den.out"))[0]
+    psb = read_mode2(os.path.join(part, "den.out"))[0]
+    assert np.array_equal(psa, psb)
