@@ -133,7 +133,8 @@ __global__ void __launch_bounds__(R* TL) k_xreg(XPassArgs<T> a) {
     const bool va = la < nlines, vb = lb < nlines;
     C* M = lds + l * MS;
     C v[R];
-    __syncthreads();
+    // the line loads are issued before the barrier that publishes the
+    // twiddle table, so the two memory latencies overlap
     if (a.mode == XMODE_R2C) {
 #pragma unroll
         for (int j = 0; j < R; ++j) {
@@ -154,6 +155,7 @@ __global__ void __launch_bounds__(R* TL) k_xreg(XPassArgs<T> a) {
             v[j] = cmk<C>(A.x - B.y, A.y + B.x);
         }
     }
+    __syncthreads();
     const bool inv = a.mode != XMODE_R2C;
     // columns -> rows: v[k2] = Z[r + R k2] (r = c)
     col_phase<C, R>(v, c, M, RS, 1, tw, inv);

@@ -131,7 +131,7 @@ __global__ void __launch_bounds__((P > Q ? P : Q) * TL) k_x2(XPassArgs<T> a) {
     C* M = lds + l * MS;
     auto pad = [](int i) { return i + (i >> 4); };  // natural order, one pad per 16
     C v[L];
-    __syncthreads();
+    // line loads first: they overlap the twiddle-table fill (barrier below)
     if (c < Q) {
         if (a.mode == XMODE_R2C) {
 #pragma unroll
@@ -154,6 +154,7 @@ __global__ void __launch_bounds__((P > Q ? P : Q) * TL) k_x2(XPassArgs<T> a) {
             }
         }
     }
+    __syncthreads();
     const bool inv = a.mode != XMODE_R2C;
     if (c < Q) phase_a<C, P, Q, 1, SA>(v, c, M, tw, inv);
     __syncthreads();

@@ -3,7 +3,7 @@
 !
 ! Reads the reference's namelist &input from stdin (DFT4He3d.f90:149-168;
 ! every key is accepted, the ones that do not concern this path are read and
-! ignored) and a mode-2 state file (readenc.f90:104-113), then runs the
+! ignored) and a state file of mode 0, 2 or 3 (readenc.f90:80-129), then runs the
 ! reference loop (DFT4He3d.f90:834-1004): Runge-Kutta-Gill for the first
 ! three iterations (or always with lrk), Hamming predictor-corrector after,
 ! the mean field and impurity force at the new state, the impurity R/v/a
@@ -102,7 +102,8 @@ program hetd_drive
       error stop 'Inconsistent deltat - deltatps'
     end if
   end if
-  if (mode /= 2) error stop 'hetd_drive: only mode=2 (readenc.f90:104-113) state files are read'
+  ! readenc.f90:136-140 (modes 0, 2 and 3 are read)
+  if (mode /= 0 .and. mode /= 2 .and. mode /= 3) error stop 'ERROR READEN: This mode is still not programmed'
   if (lsurf) error stop 'hetd_drive: lsurf (external surface potential) is not part of this path'
   if (lsolid) core4 = 'OT '
   if (core4 /= 'OT ' .and. core4 /= 'OTC') error stop ' ... The program stops due to a severe error.'
@@ -122,9 +123,15 @@ program hetd_drive
   call hetd_setup(p, selec_gs, r_cutoff_gs, umax_gs)
   call fftini(nx, ny, nz, nthread)
 
-  ! readenc mode 2 (readenc.f90:104-113)
+  ! the impurity of the namelist (DFT4He3d.f90:300-302); modes 2 and 3 read it from the file
+  rimp = (/ximp, yimp, zimp/)
+  vimp = (/vximp, vyimp, vzimp/)*ktops
   allocate(psi(nx,ny,nz))
-  call read_mode2(filedenin)
+  if (mode == 0) then
+    call read_mode0(filedenin)
+  else
+    call read_mode23(filedenin)
+  end if
   rc = hetd_set_state(hctx, psi, rimp, vimp)
   call hetd_check(rc, 'hetd_set_state')
 
@@ -264,26 +271,88 @@ contains
     name = 'density.'//chariter//'.dat'
   end function density_name
 
-  subroutine read_mode2(fname)
-    character(len=*), intent(in) :: fname
-    real(8) :: xmaxp, ymaxp, zmaxp, hxp, hyp, hzp
-    integer :: nxp, nyp, nzp, ios
+  subroutine titols(iu)
+    ! skip the leading '#' comment lines of a state file (titols.f90:12-42)
+    integer, intent(in) :: iu
     character(len=1) :: c
-    open(1, file=fname, status='old')
-    ! titols (titols.f90:12-42): skip the leading '#' lines
+    integer :: ios
     do
-      read(1, '(A1)', iostat=ios) c
+      read(iu, '(A1)', iostat=ios) c
       if (ios /= 0) error stop 'STOP due to severe errors in routine TITOLS'
       if (c /= '#') exit
     end do
-    backspace(1)
-    read(1, *) xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, nxp, nyp, nzp
+    backspace(iu)
+  end subroutine titols
+
+  subroutine check_grid(nxp, nyp, nzp)
+    integer, intent(in) :: nxp, nyp, nzp
     if (nxp /= nx .or. nyp /= ny .or. nzp /= nz) error stop 'hetd_drive: state file grid differs from the namelist'
+  end subroutine check_grid
+
+  subroutine read_mode23(fname)
+    ! readenc modes 2 and 3 (readenc.f90:104-129): grid line, rimp, vimp,
+    ! [mode 3: ninvar and the invar values, unused on this path], psi
+    character(len=*), intent(in) :: fname
+    real(8) :: xmaxp, ymaxp, zmaxp, hxp, hyp, hzp
+    real(8), allocatable :: invar(:)
+    integer :: nxp, nyp, nzp, ninvar
+    open(1, file=fname, status='old')
+    call titols(1)
+    read(1, *) xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, nxp, nyp, nzp
+    call check_grid(nxp, nyp, nzp)
     read(1, *) rimp
     read(1, *) vimp
+    if (mode == 3) then
+      read(1, *) ninvar
+      allocate(invar(ninvar))
+      read(1, *) invar
+    end if
     read(1, *) psi
     close(1)
-  end subroutine read_mode2
+  end subroutine read_mode23
+
+  subroutine read_mode0(fname)
+    ! readenc mode 0 (readenc.f90:80-100): a density, or else a complex
+    ! wave function, after a grid line that also carries limp and an
+    ! impurity position (read and not used: the namelist's stays).  A
+    ! density becomes psi = sqrt(den) exp(i k.r), the boost of
+    ! DFT4He3d.f90:606-632.  The reference forms the phase with cmplx()
+    ! of default kind, so its cosine and sine are rounded to REAL(4);
+    ! the same rounding is kept here.
+    character(len=*), intent(in) :: fname
+    real(8) :: xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, xf, yf, zf, ph, xx, yy, zz
+    real(8), allocatable :: den(:,:,:)
+    integer :: nxp, nyp, nzp, ios, ix, iy, iz
+    logical :: limpf
+    allocate(den(nx,ny,nz))
+    open(1, file=fname, status='old')
+    call titols(1)
+    read(1, *) xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, nxp, nyp, nzp, limpf, xf, yf, zf
+    call check_grid(nxp, nyp, nzp)
+    read(1, *, iostat=ios) den
+    if (ios /= 0) then
+      rewind(1)
+      call titols(1)
+      read(1, *) xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, nxp, nyp, nzp, limpf, xf, yf, zf
+      read(1, *) psi
+      write(6, '("From Readenc: We have read a complex w.f.")')
+    else
+      write(6, '("From Readenc: We have read a density")')
+      if (xlamda /= 0.d0) xlamdaz = xlamda*ktops/(2.d0*h2o2m4)
+      do iz = 1, nz
+        zz = -zmax + hz*(iz - 1)
+        do iy = 1, ny
+          yy = -ymax + hy*(iy - 1)
+          do ix = 1, nx
+            xx = -xmax + hx*(ix - 1)
+            ph = xx*xlamdax + yy*xlamday + zz*xlamdaz
+            psi(ix, iy, iz) = sqrt(den(ix, iy, iz))*cmplx(real(cos(ph), 4), real(sin(ph), 4), kind=8)
+          end do
+        end do
+      end do
+    end if
+    close(1)
+  end subroutine read_mode0
 
   subroutine moments()
     ! centre of mass (r_cm.f90:23-55), <Lx,Ly,Lz> about it (DFT4He3d.f90:883-925)

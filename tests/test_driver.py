@@ -1,5 +1,6 @@
-"""The Fortran driver bin/hetd_drive: the reference's namelist and mode-2
-files in, the reference's trajectory files and checkpoint layout out."""
+"""The Fortran driver bin/hetd_drive: the reference's namelist and state
+files (modes 0, 2, 3) in, the reference's trajectory files and checkpoint
+layout out."""
 import os
 import subprocess
 
@@ -39,7 +40,7 @@ def test_driver_built():
 
 @pytest.mark.parametrize("over,msg", [(dict(nx=7), "NX is not correct"),
                                       (dict(deltat="1.d-4"), "Inconsistent deltat"),
-                                      (dict(mode=0), "only mode=2")])
+                                      (dict(mode=4), "This mode is still not programmed")])
 def test_driver_rejects_like_the_reference(tmp_path, over, msg):
     # these checks happen on the host before any device call
     cfg, _ = G.load("ot_xe_32")
@@ -93,3 +94,92 @@ def test_driver_restart_with_history_is_bitwise(tmp_path):
     psa = read_mode2(os.path.join(full, "den.out"))[0]
     psb = read_mode2(os.path.join(part, "den.out"))[0]
     assert np.array_equal(psa, psb)
+
+
+def write_mode0(path, arr, cfg, rimp_file):
+    """readenc mode 0 input (readenc.f90:89-99): '#' titles, a grid line with
+    limp and an impurity position, then a density (real) or a wave function."""
+    nz, ny, nx = arr.shape
+    h = [2.0 * abs(cfg[k]) / n for k, n in (("xmax", nx), ("ymax", ny), ("zmax", nz))]
+    flat = np.ascontiguousarray(arr).reshape(-1)
+    with open(path, "w") as f:
+        f.write("#  static density\n#  \n")
+        f.write(" ".join(repr(float(v)) for v in (cfg["xmax"], cfg["ymax"], cfg["zmax"], *h)))
+        f.write(" %d %d %d T %r %r %r\n" % (nx, ny, nz, *map(float, rimp_file)))
+        if np.iscomplexobj(flat):
+            for i in range(0, flat.size, 3):
+                f.write(" ".join("(%r,%r)" % (float(z.real), float(z.imag)) for z in flat[i:i + 3]) + "\n")
+        else:
+            for i in range(0, flat.size, 5):
+                f.write(" ".join(repr(float(v)) for v in flat[i:i + 5]) + "\n")
+
+
+@pytest.mark.gpu
+def test_driver_mode0_density_boost_matches_mode2(tmp_path):
+    """mode 0 from a density with the boost of DFT4He3d.f90:606-632 (phase
+    rounded to REAL(4) as the reference's default-kind cmplx) runs as mode 2
+    from the boosted wave function built here."""
+    cfg, d = G.load("ot_xe_32")
+    den = np.abs(d["psi0"]) ** 2
+    kx, ky, lam = 0.31, -0.12, 0.8
+    h2o2m4 = 6.05969638298
+    kz = lam * 7.63822291 / (2.0 * h2o2m4)
+    nz, ny, nx = den.shape
+    ax = [-cfg[k] + 2.0 * abs(cfg[k]) / n * np.arange(n) for k, n in (("xmax", nx), ("ymax", ny), ("zmax", nz))]
+    ph = ax[0][None, None, :] * kx + ax[1][None, :, None] * ky + ax[2][:, None, None] * kz
+    psi = np.sqrt(den) * (np.cos(ph).astype(np.float32).astype(np.float64)
+                          + 1j * np.sin(ph).astype(np.float32).astype(np.float64))
+    r0, v0 = np.array(d["rimp0"]), np.array(d["vimp0"])
+    m0, m2 = os.path.join(tmp_path, "m0"), os.path.join(tmp_path, "m2")
+    os.makedirs(m0)
+    os.makedirs(m2)
+    write_mode0(os.path.join(m0, "den.in"), den, cfg, (9.0, 9.0, 9.0))
+    write_mode2(os.path.join(m2, "den.in"), psi, cfg["xmax"], cfg["ymax"], cfg["zmax"], r0, v0)
+    vps = v0 / 7.63822291
+    common = dict(niter=6, xlamdax=repr(kx), xlamday=repr(ky), xlamda=repr(lam))
+    a = run_drive(m0, namelist(cfg, mode=0, ximp=repr(r0[0]), yimp=repr(r0[1]), zimp=repr(r0[2]),
+                               vximp=repr(vps[0]), vyimp=repr(vps[1]), vzimp=repr(vps[2]), **common))
+    assert a.returncode == 0, a.stderr
+    assert "We have read a density" in a.stdout
+    b = run_drive(m2, namelist(cfg, **common))
+    assert b.returncode == 0, b.stderr
+    ra = np.loadtxt(os.path.join(m0, "rimp.out"), comments="#")
+    rb = np.loadtxt(os.path.join(m2, "rimp.out"), comments="#")
+    assert np.abs(ra - rb).max() < 1e-12 * np.abs(rb).max()
+    pa = read_mode2(os.path.join(m0, "den.out"))[0]
+    pb = read_mode2(os.path.join(m2, "den.out"))[0]
+    assert np.abs(pa - pb).max() < 1e-12 * np.abs(pb).max()
+
+
+@pytest.mark.gpu
+def test_driver_mode0_wavefunction_and_mode3(tmp_path):
+    """mode 0 with a complex wave function (read after the density read
+    fails, readenc.f90:89-99) and mode 3 (extra invar lines) give the mode-2
+    run bit for bit."""
+    cfg, d = G.load("ot_xe_32")
+    r0, v0 = np.array(d["rimp0"]), np.array(d["vimp0"])
+    dirs = {k: os.path.join(tmp_path, k) for k in ("m0", "m2", "m3")}
+    for w in dirs.values():
+        os.makedirs(w)
+    write_mode0(os.path.join(dirs["m0"], "den.in"), d["psi0"], cfg, (1.0, 2.0, 3.0))
+    write_mode2(os.path.join(dirs["m2"], "den.in"), d["psi0"], cfg["xmax"], cfg["ymax"], cfg["zmax"], r0, v0)
+    with open(os.path.join(dirs["m2"], "den.in")) as f:
+        lines = f.read().splitlines(True)
+    k = next(i for i, t in enumerate(lines) if not t.startswith("#"))
+    with open(os.path.join(dirs["m3"], "den.in"), "w") as f:
+        f.writelines(lines[:k + 3] + [" 3\n", " 0.5 1.5 2.5\n"] + lines[k + 3:])
+    # mode 0 takes the impurity from the namelist (velocity in A/ps)
+    vps = v0 / 7.63822291
+    out = {}
+    for m, w in dirs.items():
+        over = dict(niter=5, mode=int(m[1]))
+        if m == "m0":
+            over.update(ximp=repr(r0[0]), yimp=repr(r0[1]), zimp=repr(r0[2]),
+                        vximp=repr(vps[0]), vyimp=repr(vps[1]), vzimp=repr(vps[2]))
+        r = run_drive(w, namelist(cfg, **over))
+        assert r.returncode == 0, r.stderr
+        out[m] = (np.loadtxt(os.path.join(w, "rimp.out"), comments="#"), read_mode2(os.path.join(w, "den.out"))[0])
+    assert np.array_equal(out["m3"][0], out["m2"][0]) and np.array_equal(out["m3"][1], out["m2"][1])
+    # v0/7.638... * 7.638... may differ from v0 in the last bit
+    assert np.abs(out["m0"][0] - out["m2"][0]).max() < 1e-12 * np.abs(out["m2"][0]).max()
+    assert np.abs(out["m0"][1] - out["m2"][1]).max() < 1e-12 * np.abs(out["m2"][1]).max()

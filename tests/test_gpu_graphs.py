@@ -76,3 +76,27 @@ def test_pool_matches_sequential():
             same_state(a, b)
     for c in seq + pool:
         c.close()
+
+
+@pytest.mark.parametrize("shape,prec,icon", [((36, 40, 32), capi.HETD_FP64, 13),
+                                             ((32, 32, 40), capi.HETD_MIXED, 8)])
+def test_history_restart_is_bitwise(shape, prec, icon):
+    """A context rebuilt from psi and the saved integrator history continues
+    bit for bit as the context it was saved from (graph replays included)."""
+    say("history", shape, prec, icon)
+    a = make(shape, prec, icon, graphs=True)
+    a.evolve(G.DT, 11, traj=False)
+    blob = a.save_history()
+    psi = a.field(capi.F_PSI)
+    r, v = a.imp()[:2]
+    b = make(shape, prec, icon, graphs=True, seed=9)
+    b.set_state(psi, r, v)
+    b.load_history(blob)
+    for n in (7, 12):
+        a.evolve(G.DT, n, traj=False)
+        b.evolve(G.DT, n, traj=False)
+        same_state(a, b)
+    with pytest.raises(RuntimeError):
+        b.load_history(blob[:-8])
+    a.close()
+    b.close()
