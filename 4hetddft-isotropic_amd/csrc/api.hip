@@ -10,6 +10,7 @@
 
 #include "../../include/hetd.h"
 #include "hetd_kernels.h"
+#include "impurity.hpp"
 #include "setup.hpp"
 
 using namespace hetd;
@@ -792,9 +793,7 @@ void* hetd_stream(hetd_ctx* c) { return c ? (void*)c->st : nullptr; }
 
 int hetd_vimp(const char* selec, double r, double r_cutoff, double umax, double* v) {
     if (!selec || !v) return HETD_ERR_ARG;
-    std::string s(selec);
-    while (!s.empty() && s.back() == ' ') s.pop_back();
-    return setup::select_pot(s, r, r_cutoff, umax, v) ? HETD_OK : HETD_ERR_ARG;
+    return imp::select_pot(selec, r, r_cutoff, umax, v) ? HETD_OK : HETD_ERR_ARG;
 }
 
 int hetd_build_potion(const char* selec, double r_cutoff, double umax, int32_t nmax, double xmaxbox, double* potion,

@@ -144,41 +144,6 @@ inline double pgrid(int k, int n, double h) {
     return (k + 1 <= n / 2 + 1) ? hp * k : -pmax + hp * (k + 1 - (n / 2 + 1));
 }
 
-// Tang-Toennies Xe-He potential with the reference's constants, evaluated in
-// double in the reference's order (V_impur.f90:2728-2772).
-inline double v_xe_he(double r) {
-    const double AA = 74.9002, BB = 1.8175, rbohr = 0.5292, h2k = 27.21 * 11604.0;
-    double B[19], C[8], S[19];
-    B[1] = BB;
-    for (int i = 2; i <= 18; ++i) B[i] = B[i - 1] * BB / (double)i;
-    C[1] = 21.5674;
-    C[2] = 430.148;
-    C[3] = 10610.21;
-    for (int i = 4; i <= 7; ++i) C[i] = std::pow(C[i - 1] / C[i - 2], 3) * C[i - 3];
-    const double rr = r / rbohr, e = std::exp(-BB * rr);
-    double st = rr;
-    S[1] = B[1] * st;
-    for (int i = 2; i <= 18; ++i) {
-        st = st * rr;
-        S[i] = S[i - 1] + B[i] * st;
-    }
-    double aux = AA * e;
-    const double s1 = 1.0 / (rr * rr);
-    st = s1 * s1 * s1;
-    for (int i = 1; i <= 7; ++i) {
-        aux = aux + ((S[2 * i + 4] + 1.0) * e - 1.0) * C[i] * st;
-        st = st * s1;
-    }
-    return aux * h2k;
-}
-
-// Select_Pot (V_impur.f90:84-929) for the supported selec strings.
-inline bool select_pot(const std::string& selec, double r, double r_cutoff, double umax, double* v) {
-    if (selec == "null") { *v = 0.0; return true; }
-    if (selec == "Xe_He") { *v = r < r_cutoff ? umax : v_xe_he(r); return true; }
-    return false;
-}
-
 template <typename F> inline void parallel_for(int n, F f) {
     unsigned nt = std::thread::hardware_concurrency();
     if (nt == 0) nt = 1;

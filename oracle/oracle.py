@@ -93,15 +93,21 @@ def build_potion(selec, r_cutoff, umax, nmax, xmaxbox):
 class OracleCtx:
     """Same methods as hetd.Context, computed by the C restatement."""
 
-    def __init__(self, params: "capi.Params", selec="null", r_cutoff=2.0, umax=0.0):
+    def __init__(self, params: "capi.Params", selec="null", r_cutoff=2.0, umax=0.0, potion=None):
+        """potion: (table, spacing) of V(r) given from outside (a potential the
+        oracle does not restate; the table itself is pinned against the
+        reference's Select_Pot on its own), else built from the oracle's V."""
         self.L = lib()
         self.p = params
         self.h = self.L.orc_create(C.byref(params))
         self.shape = (params.nz, params.ny, params.nx)
         self.n = params.nx * params.ny * params.nz
         self.nh = (params.nx // 2 + 1) * params.ny * params.nz
-        tab, d = build_potion(selec, r_cutoff, umax, max(params.nx, params.ny, params.nz),
-                              max(params.xmax, params.ymax, params.zmax))
+        if potion is not None:
+            tab, d = np.ascontiguousarray(potion[0], dtype=np.float64), float(potion[1])
+        else:
+            tab, d = build_potion(selec, r_cutoff, umax, max(params.nx, params.ny, params.nz),
+                                  max(params.xmax, params.ymax, params.zmax))
         self.potion, self.delinter = tab, d
         self._chk(self.L.orc_set_potion(self.h, _p(tab), len(tab), d))
 

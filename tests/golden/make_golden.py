@@ -14,7 +14,9 @@ For every configuration below it
      correct implementation, and three planes keep the fixtures small).
 It also extracts the centred rows of the reference's finite-difference table
 (blockds.f) for M = 3..25, K = 1, 2 into fd_coeffs.json, and the two
-Tang-Toennies Xe-He values quoted in V_impur.f90:759-760 and the sample input.
+Tang-Toennies Xe-He values quoted in V_impur.f90:759-760 and the sample input,
+and Select_Pot of every impurity potential on a grid of distances
+(impurity_potentials.npz, from oracle/_ref/ref_pots).
 """
 from __future__ import annotations
 
@@ -164,8 +166,37 @@ def fd_table():
     print("fd_coeffs.json", len(out))
 
 
+# every name Select_Pot accepts (V_impur.f90:84-929)
+POTENTIALS = ("null Aziz_He He2S_fci He_plus LJ_OT LI NA K RB CS Cs_plus_Koutselos Na_plus_Koutselos "
+              "Rb_plus_Koutselos Rb_plus_Fausto K_plus_Koutselos Ba_gs Ba_plus_old_gs Ba_plus_gs "
+              "Ba_plus_gs_fix_C4 Ba_plus_pi Ba_plus_pi_fix_C4 Ba_plus_sigma Ba_plus_sigma_fix_C4 Ba_plus_2D0 "
+              "Ba_plus_2D1 Ba_plus_2D2 Ag_gs Ag_Pi Ag_Sig Cs_7s Rb_6s Cs_sigma Cs_pi Cs_gs_Fausto "
+              "Cs_6p_Sigma_Fausto Cs_6p_Pi_Fausto Cs_7s_Fausto Rb_sigma Rb_pi Rb_6p_sigma Rb_6p_pi Rb_gs_Fausto "
+              "Rb_5p_Sigma_Fausto Rb_5p_Pi_Fausto Rb_6s_Fausto Grafeno Grafeno_sin_dispersion TiO2 Au Au_TiO2 "
+              "dAu_TiO2 pH2_He Ag_Graphene dz_Ag_Graphene Xe_He Ar_He Au_Graphene dz_Au_Graphene CH3I_0 CH3I_1 "
+              "CH3I_2 CH3I_3 CH3I_4 K_4p_Sigma K_4p_Pi K_5s").split()
+
+
+def potentials():
+    """Select_Pot of the reference (oracle/_ref/ref_pots) for every name on
+    a dense grid of distances plus the branch points of the fits (inner
+    cut-offs, quadratic caps, the 2.5 A clamp of Cs_7s, the 3.6 / 3.6f
+    cut-offs of the CH3I components)."""
+    r = np.concatenate([np.linspace(0.3, 25.0, 495),
+                        [0.529177, 0.74, 1.0, 2.0, 2.1, 2.2, 2.35, 2.5, 3.0, 3.1750200000000000, 3.5999999,
+                         3.6, 3.7, 4.124]])
+    with tempfile.TemporaryDirectory() as work:
+        inp = "%d\n%s\n%s\n" % (len(r), " ".join(repr(float(x)) for x in r), "\n".join(POTENTIALS))
+        subprocess.run([os.path.join(ROOT, "oracle", "_ref", "ref_pots")], input=inp, text=True, cwd=work,
+                       check=True, stdout=subprocess.DEVNULL)
+        v = np.fromfile(os.path.join(work, "pots.bin"), dtype="<f8").reshape(len(POTENTIALS), len(r))
+    np.savez_compressed(os.path.join(HERE, "impurity_potentials.npz"), r=r, names=np.array(POTENTIALS), v=v)
+    print("impurity_potentials.npz", v.shape)
+
+
 def main():
     fd_table()
+    potentials()
     kat = {"V_Xe_He": [[2.0, 21450.4867067095], [2.55, 2793.9365463]],
            "b4": -718.99, "source": "V_impur.f90:759-760, input_files sample umax, modules.f90:212"}
     with open(os.path.join(HERE, "kat.json"), "w") as f:

@@ -184,6 +184,35 @@ def test_evolve_matches_oracle_other_grids(oracle_mod, shape, npd, icon, core):
         assert abs(e1[k] - e2[k]) <= TOL * max(abs(e2[k]), 1.0), k
 
 
+@pytest.mark.parametrize("selec,r_cutoff,umax", [("RB", 3.0, 1.0e3), ("Ba_plus_gs", 1.05, 6.836177e3),
+                                                 ("Ag_gs", 2.0, 5.0e3)])
+def test_evolve_other_impurity_potentials(oracle_mod, selec, r_cutoff, umax):
+    """Other impurities of Select_Pot: the table (pinned bitwise against the
+    reference's Select_Pot in test_impurity.py) is handed to both sides."""
+    say("impurity", selec)
+    from hetd import build_potion
+    from hetd.initstate import woods_saxon_state
+    nz, ny, nx = 36, 40, 48
+    h = 0.6
+    p = capi.default_params(nx=nx, ny=ny, nz=nz, xmax=nx * h / 2, ymax=ny * h / 2, zmax=nz * h / 2, npd=13,
+                            icon=13)
+    tab = build_potion(selec, r_cutoff, umax, max(nx, ny, nz), max(p.xmax, p.ymax, p.zmax))
+    ctx = Context(p, potion=tab)
+    orc = oracle_mod.OracleCtx(p, potion=tab)
+    psi = woods_saxon_state(nx, ny, nz, p.xmax, p.ymax, p.zmax, 150, 3)
+    for c in (ctx, orc):
+        c.set_state(psi, (0.5, -0.2, 3.0), (0.0, 1.0, -2.0))
+        c.init()
+    t1 = ctx.evolve(G.DT, 6)
+    t2 = orc.evolve(G.DT, 6)
+    assert np.abs(t1[:, 2:] - t2[:, 2:]).max() <= TOL * np.abs(t2[:, 2:]).max()
+    assert G.rel(ctx.field(capi.F_PSI), orc.field(capi.F_PSI)) < TOL
+    e1, e2 = ctx.energy(), orc.energy()
+    for k in e2:
+        assert abs(e1[k] - e2[k]) <= TOL * max(abs(e2[k]), 1.0), k
+    ctx.close()
+
+
 @pytest.mark.parametrize("npd,icon,prec", [(13, 13, capi.HETD_FP64), (9, 8, capi.HETD_FP64), (7, 13, capi.HETD_FP64),
                                            (13, 13, capi.HETD_MIXED)])
 def test_moments_match_oracle(oracle_mod, npd, icon, prec):
