@@ -58,6 +58,9 @@ struct hetd_ctx {
     float2 *A32 = nullptr, *B32 = nullptr, *C32 = nullptr;
     float *fvlj32 = nullptr, *wcgk32 = nullptr, *kalfs32 = nullptr;
     double* potion = nullptr;
+    long partial_len = 0;    // doubles in partial
+    double* uext = nullptr;  // external potential of the energy (lsurf), or none
+    bool uext_on = false;
     int npot = 0;
     double delinter = 0;
     ImpState* imp = nullptr;
@@ -932,6 +935,7 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
     HCHK(c, dalloc(c, &c->guard, sizeof(int)));
     const long nblocks = (long)((c->nx + 63) / 64) * ((c->ny + 3) / 4) * c->nz;
     HCHK(c, dalloc(c, &c->partial, sizeof(double) * nblocks * EN_NCOMP));
+    c->partial_len = nblocks * EN_NCOMP;
     HCHK(c, dalloc(c, &c->red, sizeof(double) * 16));
     HCHK(c, hipMemset(c->imp, 0, sizeof(ImpState)));
     HCHK(c, hipMemset(c->guard, 0, sizeof(int)));
@@ -1024,6 +1028,44 @@ int hetd_set_potion(hetd_ctx* c, const double* potion, int32_t npot, double deli
     fill_grid(c);
     drop_graph(c);  // the captured launches hold the old grid arguments
     return HETD_OK;
+}
+
+int hetd_vsurf(int32_t lsurf3d, double x, double y, double z, double* v) {
+    if (!v) return HETD_ERR_ARG;
+    *v = lsurf3d ? imp::morse_tio2_3d(x, y, z) : imp::morse_tio2_1d(z);
+    return HETD_OK;
+}
+
+int hetd_set_uext(hetd_ctx* c, const double* uext) {
+    if (!c) return HETD_ERR_ARG;
+    if (!uext) {
+        c->uext_on = false;
+        return HETD_OK;
+    }
+    if (!c->uext) HCHK(c, dalloc(c, &c->uext, sizeof(double) * c->n));
+    HCHK(c, hipMemcpyAsync(c->uext, uext, sizeof(double) * c->n, hipMemcpyHostToDevice, c->st));
+    HCHK(c, hipStreamSynchronize(c->st));
+    c->uext_on = true;
+    return HETD_OK;
+}
+
+int hetd_set_surface(hetd_ctx* c, int32_t lsurf3d, double zsurf) {
+    if (!c) return HETD_ERR_ARG;
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    const hetd_params& p = c->p;
+    std::vector<double> u((size_t)c->n);
+    setup::parallel_for(nz, [&](int k) {
+        const double z = -p.zmax + c->hz * k;
+        for (int j = 0; j < ny; ++j) {
+            const double y = -p.ymax + c->hy * j;
+            for (int i = 0; i < nx; ++i) {
+                const double x = -p.xmax + c->hx * i;
+                const double v = lsurf3d ? imp::morse_tio2_3d(x, y, z - zsurf) : imp::morse_tio2_1d(z - zsurf);
+                u[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] = std::min(3000.0, v);
+            }
+        }
+    });
+    return hetd_set_uext(c, u.data());
 }
 
 int hetd_set_state(hetd_ctx* c, const double* psi, const double r[3], const double v[3]) {
@@ -1278,6 +1320,15 @@ int hetd_energy(hetd_ctx* c, hetd_energies* e) {
     e->ecor4 = s[EN_COR] * c->dxyz;
     e->etot4 = e->ekin4 + e->elj4 + e->ecor4 + e->esolid;
     if (c->otc) e->etot4 = e->etot4 + e->ealphas;
+    if (c->uext_on) {  // energy.f90:124
+        const int cap = (int)std::min<long>(1024, c->partial_len);
+        HCHK(c, launch_dot_uext(c->psi, c->uext, c->n, cap, c->partial, c->st));
+        HCHK(c, launch_reduce_final(c->partial, cap, 1, c->red + 8, c->st));
+        double su = 0.0;
+        HCHK(c, hipMemcpyAsync(&su, c->red + 8, sizeof su, hipMemcpyDeviceToHost, c->st));
+        HCHK(c, hipStreamSynchronize(c->st));
+        e->etot4 = e->etot4 + su * c->dxyz;
+    }
     e->ekinx = 0.5 * c->mAg * (is.v[0] * is.v[0] + is.v[1] * is.v[1] + is.v[2] * is.v[2]);
     e->eHeX = s[EN_HEX] * c->dxyz;
     e->eimpu = e->ekinx + e->eHeX;

@@ -238,6 +238,8 @@ hipError_t launch_energy_tiled(const EnergyArgs& a, hipStream_t st, int* nblocks
 
 // out[c] = sum_b partial[b*ncomp + c], fixed order (bitwise reproducible)
 hipError_t launch_reduce_final(const double* partial, int nblocks, int ncomp, double* out, hipStream_t st);
+// partial[b] = sum over a fixed slice of u * |psi|^2, nblocks blocks (<= 1024)
+hipError_t launch_dot_uext(const double2* psi, const double* u, long n, int nblocks, double* partial, hipStream_t st);
 
 // ------------------------------------------------------------ impurity ODE
 

@@ -43,6 +43,27 @@ hipError_t launch_reduce_final(const double* partial, int nblocks, int ncomp, do
     return hipGetLastError();
 }
 
+// sum(uext * den) of energy.f90:124: each block sums a fixed strided slice
+// and a fixed tree, so the result is the same run to run
+__global__ void __launch_bounds__(256) k_dot_uext(const double2* psi, const double* u, long n, double* partial) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) s += u[i] * abs2(psi[i]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+hipError_t launch_dot_uext(const double2* psi, const double* u, long n, int nblocks, double* partial, hipStream_t st) {
+    if (nblocks < 1 || nblocks > 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_dot_uext, dim3(nblocks), dim3(256), 0, st, psi, u, n, partial);
+    return hipGetLastError();
+}
+
 // F = -sum(grad den * u_imp) * dxyz (potenimp.f90:44-46); aimp = F/mAg_u
 // (DFT4He3d.f90:852)
 __global__ void __launch_bounds__(256) k_force_final(ImpState* imp, const double* partial, int nblocks, int stride,

@@ -375,5 +375,42 @@ const char* known_potentials() {
     return names.c_str();
 }
 
+// He on TiO2, laterally averaged: De (e^{2 alpha (Re - z)} - 2 e^{alpha (Re - z)})
+// in cm^-1, times 1.4387752 K/cm^-1 written as a default-real literal
+double morse_tio2_1d(double z) {
+    const double De = 39.95, alpha = 1.721, Re = 4.3619;
+    return De * (std::exp(2.0 * alpha * (Re - z)) - 2.0 * std::exp(alpha * (Re - z))) * (double)1.4387752f;
+}
+
+// He on TiO2 with the surface corrugation: De, alpha and Ze are each
+// Lx^T M Ly with Lx = cos(2 pi [0,1,2] x/a), Ly = cos(2 pi [0,1,2,3] y/b)
+// (products summed in the order of the matmul calls); K
+double morse_tio2_3d(double x, double y, double z) {
+    static const double kDe[3][4] = {{54.671500, -2.228480, 0.585194, 13.211000},
+                                     {4.842040, 0.363525, 4.614530, -1.182030},
+                                     {3.747220, 3.222690, 0.333350, 0.587320}};
+    static const double kAlpha[3][4] = {{1.492420, -0.010921, 0.003189, -0.168927},
+                                        {-0.097618, -0.024780, 0.062441, 0.016801},
+                                        {-0.034535, 0.030015, 0.012846, -0.011699}};
+    static const double kZe[3][4] = {{4.073990, 0.057537, -0.001500, -0.568990},
+                                     {-0.115578, -0.018127, -0.090046, 0.008590},
+                                     {-0.075153, -0.051210, -0.023349, -0.017931}};
+    const double pi = std::acos(-1.0), a = 2.9850, b = 6.49548;
+    double lx[3], ly[4];
+    for (int i = 0; i < 3; ++i) lx[i] = std::cos(2.0 * pi * (double)i * x / a);
+    for (int j = 0; j < 4; ++j) ly[j] = std::cos(2.0 * pi * (double)j * y / b);
+    auto form = [&](const double (*M)[4]) {
+        double s = 0.0;
+        for (int j = 0; j < 4; ++j) {
+            double t = 0.0;
+            for (int i = 0; i < 3; ++i) t = t + lx[i] * M[i][j];
+            s = s + t * ly[j];
+        }
+        return s;
+    };
+    const double De = form(kDe), al = form(kAlpha), Ze = form(kZe);
+    return De * (std::exp(-(2.0 * al * (z - Ze))) - 2.0 * std::exp(-(al * (z - Ze))));
+}
+
 }  // namespace imp
 }  // namespace hetd

@@ -1,5 +1,6 @@
 // impurity.hpp -- the impurity-helium pair potentials V(r) (r in Angstrom,
-// V in K) that fill the impurity table of potenimpini (potenimpini.f90:11-25).
+// V in K) that fill the impurity table of potenimpini (potenimpini.f90:11-25),
+// and the He-surface potential of the lsurf option.
 // Host code only (impurity.cpp, built by the host C++ compiler: several fits
 // are summed in binary128 as the reference does).
 #pragma once
@@ -15,6 +16,11 @@ bool select_pot(const char* selec, double r, double r_cutoff, double umax, doubl
 
 // Names known to select_pot, as one blank-separated string.
 const char* known_potentials();
+
+// He-TiO2 surface Morse potentials, K: Morse_HeTiO2_1D(z) (morse.f90:1-16)
+// and Morse_HeTiO2_3D(x, y, z) (morse.f90:36-99)
+double morse_tio2_1d(double z);
+double morse_tio2_3d(double x, double y, double z);
 
 }  // namespace imp
 }  // namespace hetd

@@ -87,6 +87,22 @@ module hetd_capi
       type(c_ptr), value :: ctx
       real(c_double), dimension(3), intent(in) :: a
     end function
+    integer(c_int) function hetd_vsurf(lsurf3d, x, y, z, v) bind(C, name='hetd_vsurf')
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), value :: lsurf3d
+      real(c_double), value :: x, y, z
+      real(c_double), intent(out) :: v
+    end function
+    integer(c_int) function hetd_set_uext(ctx, uext) bind(C, name='hetd_set_uext')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, uext
+    end function
+    integer(c_int) function hetd_set_surface(ctx, lsurf3d, zsurf) bind(C, name='hetd_set_surface')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: lsurf3d
+      real(c_double), value :: zsurf
+    end function
     integer(c_int) function hetd_steprk(ctx, dt) bind(C, name='hetd_steprk')
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx

@@ -104,7 +104,6 @@ program hetd_drive
   end if
   ! readenc.f90:136-140 (modes 0, 2 and 3 are read)
   if (mode /= 0 .and. mode /= 2 .and. mode /= 3) error stop 'ERROR READEN: This mode is still not programmed'
-  if (lsurf) error stop 'hetd_drive: lsurf (external surface potential) is not part of this path'
   if (lsolid) core4 = 'OT '
   if (core4 /= 'OT ' .and. core4 /= 'OTC') error stop ' ... The program stops due to a severe error.'
 
@@ -135,6 +134,12 @@ program hetd_drive
   rc = hetd_set_state(hctx, psi, rimp, vimp)
   call hetd_check(rc, 'hetd_set_state')
 
+  ! external potential of the TiO2 surface (DFT4He3d.f90:758-784): the
+  ! reference adds it to the energy only
+  if (lsurf) then
+    write(6, '(" Compute surface potential")')
+    call hetd_check(hetd_set_surface(hctx, merge(1, 0, lsurf3D), Zsurf), 'set_surface')
+  end if
   write(6, '(" Time step is ",1p,E16.8," kelvins or ",E16.8," picoseconds.")') deltat, deltatps
   ! first potential and energy (DFT4He3d.f90:641-645)
   call potenimp(rimp)

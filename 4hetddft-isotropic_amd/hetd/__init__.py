@@ -120,6 +120,19 @@ class Context:
         self._chk(self.L.hetd_steppc(self.h, dt, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def set_uext(self, uext):
+        """External potential of the energy (energy.f90:124); None removes it."""
+        if uext is None:
+            self._chk(self.L.hetd_set_uext(self.h, None))
+            return
+        a = np.ascontiguousarray(uext, dtype=np.float64)
+        assert a.shape == self.shape
+        self._chk(self.L.hetd_set_uext(self.h, _p(a)))
+
+    def set_surface(self, lsurf3d, zsurf):
+        """lsurf: the He-TiO2 potential as uext (DFT4He3d.f90:758-784)."""
+        self._chk(self.L.hetd_set_surface(self.h, int(lsurf3d), float(zsurf)))
+
     def energy(self):
         e = capi.Energies()
         self._chk(self.L.hetd_energy(self.h, C.byref(e)))
@@ -206,6 +219,14 @@ class Context:
         ov = out.view(np.float64) if cplx else out
         self._chk(self.L.hetd_deriv(self.h, id_, iv, int(cplx), _p(fv), _p(ov)))
         return out
+
+
+def vsurf(lsurf3d, x, y, z):
+    """He-TiO2 surface potential (morse.f90) at (x, y, z) above the surface, K."""
+    v = C.c_double()
+    if capi.load().hetd_vsurf(int(lsurf3d), float(x), float(y), float(z), C.byref(v)):
+        raise ValueError("hetd_vsurf")
+    return v.value
 
 
 def pool_evolve(contexts, dt, nsteps):

@@ -106,6 +106,9 @@ def _proto(lib):
     lib.hetd_build_potion.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_int32,
                                       C.c_double, _dp, C.POINTER(C.c_int32), _dp]
     lib.hetd_set_potion.argtypes = [_vp, _dp, C.c_int32, C.c_double]
+    lib.hetd_vsurf.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double, _dp]
+    lib.hetd_set_uext.argtypes = [_vp, _dp]
+    lib.hetd_set_surface.argtypes = [_vp, C.c_int32, C.c_double]
     lib.hetd_set_state.argtypes = [_vp, _dp, _dp, _dp]
     lib.hetd_potenimp.argtypes = [_vp, _dp]
     lib.hetd_poten.argtypes = [_vp]
@@ -133,7 +136,8 @@ def _proto(lib):
     lib.hetd_timing.restype = C.c_int
     lib.hetd_timing_read.argtypes = [_vp, _dp, C.POINTER(C.c_int64), _dp]
     lib.hetd_timing_read.restype = C.c_int
-    for name in ("hetd_init", "hetd_finalize", "hetd_vimp", "hetd_build_potion",
+    for name in ("hetd_init", "hetd_finalize", "hetd_vimp", "hetd_build_potion", "hetd_vsurf",
+                 "hetd_set_uext", "hetd_set_surface",
                  "hetd_set_potion", "hetd_set_state", "hetd_potenimp", "hetd_poten",
                  "hetd_forceimp", "hetd_set_aimp", "hetd_steprk", "hetd_steppc",
                  "hetd_energy", "hetd_norm", "hetd_moments", "hetd_save_history", "hetd_load_history",
@@ -144,7 +148,8 @@ def _proto(lib):
 
 
 EXPORTED = ("hetd_default_params", "hetd_init", "hetd_finalize", "hetd_last_error",
-            "hetd_vimp", "hetd_build_potion", "hetd_set_potion", "hetd_set_state",
+            "hetd_vimp", "hetd_build_potion", "hetd_set_potion", "hetd_vsurf", "hetd_set_uext",
+            "hetd_set_surface", "hetd_set_state",
             "hetd_potenimp", "hetd_poten", "hetd_forceimp", "hetd_set_aimp",
             "hetd_steprk", "hetd_steppc", "hetd_energy", "hetd_norm", "hetd_moments", "hetd_evolve",
             "hetd_history_size", "hetd_save_history", "hetd_load_history", "hetd_pool_evolve",

@@ -100,9 +100,10 @@ int hetd_finalize(hetd_ctx *ctx);
 const char *hetd_last_error(hetd_ctx *ctx);
 
 /* Impurity pair potential V(r) = Select_Pot(selec, r, r_cutoff, umax)
- * (V_impur.f90:84-929); selec "Xe_He" (V_impur.f90:757-770, 2728-2772) or
- * "null" (V_impur.f90:168-178).  Returns 0 and writes *v, or HETD_ERR_ARG
- * for an unknown selec (the reference STOPs, V_impur.f90:925-927). */
+ * (V_impur.f90:84-929); selec is any name Select_Pot accepts ("Xe_He",
+ * "null", "RB", "Ba_plus_gs", ...).  Returns 0 and writes *v, or
+ * HETD_ERR_ARG for an unknown selec (the reference STOPs,
+ * V_impur.f90:925-927). */
 int hetd_vimp(const char *selec, double r, double r_cutoff, double umax, double *v);
 /* Interpolation table of potenimpini (potenimpini.f90:11-25):
  * npot = 100*max(nx,ny,nz)+1, delinter = 4*max(xmax,ymax,zmax)/(npot-1),
@@ -113,6 +114,18 @@ int hetd_build_potion(const char *selec, double r_cutoff, double umax,
                       int32_t *npot, double *delinter);
 /* Upload the table (module interpol, modules.f90:363-369). */
 int hetd_set_potion(hetd_ctx *ctx, const double *potion, int32_t npot, double delinter);
+
+/* He-TiO2 surface potential (K) at a point z above the surface: the
+ * laterally averaged Morse_HeTiO2_1D(z) (morse.f90:1-16) for lsurf3d = 0,
+ * the corrugated Morse_HeTiO2_3D(x, y, z) (morse.f90:36-99) otherwise. */
+int hetd_vsurf(int32_t lsurf3d, double x, double y, double z, double *v);
+/* External potential uext (nx*ny*nz, Fortran order; NULL removes it).  As in
+ * the reference it enters only the energy: etot4 += sum(uext*den)*dxyz
+ * (energy.f90:124); the dynamics do not see it. */
+int hetd_set_uext(hetd_ctx *ctx, const double *uext);
+/* lsurf: uext = min(3000, hetd_vsurf(lsurf3d, x, y, z - zsurf)) on the grid
+ * (DFT4He3d.f90:758-784), set with hetd_set_uext. */
+int hetd_set_surface(hetd_ctx *ctx, int32_t lsurf3d, double zsurf);
 
 /* Upload psi (complex(8), nx*ny*nz) and the impurity R, v (internal units,
  * v in A*K); sets den = |psi|^2 (readenc.f90:104-113, DFT4He3d.f90:637),

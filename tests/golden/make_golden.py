@@ -189,8 +189,10 @@ def potentials():
         inp = "%d\n%s\n%s\n" % (len(r), " ".join(repr(float(x)) for x in r), "\n".join(POTENTIALS))
         subprocess.run([os.path.join(ROOT, "oracle", "_ref", "ref_pots")], input=inp, text=True, cwd=work,
                        check=True, stdout=subprocess.DEVNULL)
-        v = np.fromfile(os.path.join(work, "pots.bin"), dtype="<f8").reshape(len(POTENTIALS), len(r))
-    np.savez_compressed(os.path.join(HERE, "impurity_potentials.npz"), r=r, names=np.array(POTENTIALS), v=v)
+        v = np.fromfile(os.path.join(work, "pots.bin"), dtype="<f8").reshape(len(POTENTIALS) + 2, len(r))
+    # the last two rows: Morse_HeTiO2_1D(r) and Morse_HeTiO2_3D(0.3 r, 1.7 - 0.2 r, r) (morse.f90)
+    np.savez_compressed(os.path.join(HERE, "impurity_potentials.npz"), r=r, names=np.array(POTENTIALS),
+                        v=v[:-2], morse1d=v[-2], morse3d=v[-1])
     print("impurity_potentials.npz", v.shape)
 
 

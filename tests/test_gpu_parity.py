@@ -213,6 +213,45 @@ def test_evolve_other_impurity_potentials(oracle_mod, selec, r_cutoff, umax):
     ctx.close()
 
 
+@pytest.mark.parametrize("lsurf3d,prec", [(0, capi.HETD_FP64), (1, capi.HETD_FP64), (1, capi.HETD_MIXED)])
+def test_surface_potential_enters_the_energy_only(lsurf3d, prec):
+    """lsurf (DFT4He3d.f90:758-784, energy.f90:124): etot4 and etot gain
+    sum(uext*den)*dxyz with uext = min(3000, V_surf(x, y, z - Zsurf)); the
+    state evolves exactly as without it."""
+    say("surface", lsurf3d, prec)
+    from hetd import vsurf
+    from hetd.initstate import woods_saxon_state
+    nz, ny, nx = 40, 36, 32
+    h = 0.6
+    p = capi.default_params(nx=nx, ny=ny, nz=nz, xmax=nx * h / 2, ymax=ny * h / 2, zmax=nz * h / 2, npd=13,
+                            icon=13, precision=prec)
+    psi = woods_saxon_state(nx, ny, nz, p.xmax, p.ymax, p.zmax, 150, 3)
+    a = Context(p, selec="Xe_He", r_cutoff=2.55, umax=2793.9365463)
+    b = Context(p, selec="Xe_He", r_cutoff=2.55, umax=2793.9365463)
+    zsurf = -9.0
+    for c in (a, b):
+        c.set_state(psi, (0.3, -0.2, 0.1), (1.0, 2.0, -3.0))
+        c.init()
+    b.set_surface(lsurf3d, zsurf)
+    a.evolve(G.DT, 5, traj=False)
+    b.evolve(G.DT, 5, traj=False)
+    assert np.array_equal(a.field(capi.F_PSI), b.field(capi.F_PSI))
+    x = -p.xmax + h * np.arange(nx)
+    y = -p.ymax + h * np.arange(ny)
+    z = -p.zmax + h * np.arange(nz)
+    u = np.array([[[min(3000.0, vsurf(lsurf3d, xi, yj, zk - zsurf)) for xi in x] for yj in y] for zk in z])
+    den = np.abs(a.field(capi.F_PSI)) ** 2
+    extra = float(np.sum(u * den)) * h ** 3
+    ea, eb = a.energy(), b.energy()
+    assert abs((eb["etot4"] - ea["etot4"]) - extra) <= 1e-11 * (abs(extra) + abs(ea["etot4"]))
+    assert abs((eb["etot"] - ea["etot"]) - extra) <= 1e-11 * (abs(extra) + abs(ea["etot"]))
+    assert eb["ekin4"] == ea["ekin4"] and eb["eHeX"] == ea["eHeX"]
+    b.set_uext(None)
+    assert b.energy()["etot4"] == ea["etot4"]
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("npd,icon,prec", [(13, 13, capi.HETD_FP64), (9, 8, capi.HETD_FP64), (7, 13, capi.HETD_FP64),
                                            (13, 13, capi.HETD_MIXED)])
 def test_moments_match_oracle(oracle_mod, npd, icon, prec):

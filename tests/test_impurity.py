@@ -67,3 +67,17 @@ def test_potion_table_is_select_pot_on_its_grid():
     assert len(tab) == 6401 and d == 4 * 25.6 / 6400
     for i in (0, 1, 2, 3, 100, 999, 6400):
         assert tab[i] == vimp(L, "Ba_plus_gs", i * d, 1.05, 6.836177e3)
+
+
+def test_surface_potentials_are_bitwise():
+    """Morse_HeTiO2_1D and Morse_HeTiO2_3D (morse.f90) behind hetd_vsurf,
+    against the reference's own functions on the same points."""
+    from hetd import vsurf
+    z = np.load(os.path.join(G.GOLDEN, "impurity_potentials.npz"))
+    r = z["r"]
+    one = np.array([vsurf(0, 0.0, 0.0, x) for x in r])
+    three = np.array([vsurf(1, 0.3 * x, 1.7 - 0.2 * x, x) for x in r])
+    assert np.array_equal(one, z["morse1d"])
+    assert np.array_equal(three, z["morse3d"])
+    # the 1D form ignores x and y
+    assert vsurf(0, 3.0, -2.0, 5.0) == vsurf(0, 0.0, 0.0, 5.0)
