@@ -75,6 +75,7 @@ struct hetd_ctx {
     bool have_state = false;
     // predictor-corrector steps replayed from a HIP graph (see step_graph)
     bool use_graphs = true;
+    bool xsplit = false;  // x passes with split LDS exchanges (HETD_XSPLIT)
     hipGraphExec_t gexec = nullptr;
     int gper = 0;        // steps per replay (the period of the buffer rotation)
     double gdt = 0.0;
@@ -161,6 +162,7 @@ template <typename T> static hipError_t run_x(hetd_ctx* c, XPassArgs<T> x) {
         break;
     }
     tbeg(c, HETD_K_XPASS);
+    x.split = c->xsplit ? 1 : 0;
     hipError_t e = reg_fft_supported(x.ax.n) ? launch_xreg(x, c->st)
                    : reg2_supported(x.ax.n)  ? launch_x2(x, c->st)
                                              : launch_xpass(x, c->st);
@@ -854,6 +856,7 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
     }
     HCHK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     if (const char* e = getenv("HETD_GRAPHS")) c->use_graphs = e[0] != '0';
+    if (const char* e = getenv("HETD_XSPLIT")) c->xsplit = e[0] != '0';
     // grid tables (DFT4He3d.f90:309-317, 689-699)
     const double hh[3] = {c->hx, c->hy, c->hz}, mx[3] = {p->xmax, p->ymax, p->zmax};
     const double tmean[3] = {p->txmean, p->tymean, p->tzmean}, tsurf[3] = {p->txsurf, p->tysurf, p->tzsurf};

@@ -209,6 +209,129 @@ __global__ void __launch_bounds__(R* TL) k_xreg(XPassArgs<T> a) {
     }
 }
 
+// x pass with the LDS exchanges done in two halves (real parts, then
+// imaginary parts) through a buffer of scalars: half the LDS per line pair,
+// so more workgroups fit on a CU, for twice the barriers.  Same arithmetic
+// as k_xreg, so the same bits.
+
+// w[m] goes to slot wi(m), r[n] comes from slot ri(n); r may alias w.  Ends
+// with a barrier, so the buffer is free again on return.
+template <typename C, int NW, int NR, typename WI, typename RI>
+__device__ __forceinline__ void xchg_split(const C* w, C* r, decltype(C::x)* L, WI wi, RI ri) {
+    using T = decltype(C::x);
+    T t[NR];
+#pragma unroll
+    for (int m = 0; m < NW; ++m) L[wi(m)] = w[m].x;
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < NR; ++n) t[n] = L[ri(n)];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < NW; ++m) L[wi(m)] = w[m].y;
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < NR; ++n) r[n] = cmk<C>(t[n], L[ri(n)]);
+    __syncthreads();
+}
+
+// v[k] *= W_N^(e k) for k >= 1 (the twiddles of col_phase / rowdata_phase1)
+template <typename C, int R>
+__device__ __forceinline__ void twiddle_regs(C* v, int e, const C* tw, bool inv) {
+    constexpr int N = R * R;
+#pragma unroll
+    for (int k = 1; k < R; ++k) {
+        C w = tw[(e * k) & (N - 1)];
+        if (inv) w.y = -w.y;
+        v[k] = cmul(v[k], w);
+    }
+}
+
+template <typename T, int R, int TL>
+__global__ void __launch_bounds__(R* TL) k_xreg_s(XPassArgs<T> a) {
+    using C = cplx<T>;
+    constexpr int N = R * R, NH = N / 2 + 1, RS = R + 1, MS = R * RS, NJ = R / 2 + 1;
+    __shared__ T lds[TL * MS];
+    __shared__ C tw[N];
+    const int tid = threadIdx.x, c = tid % R, l = tid / R;
+    for (int e = tid; e < N; e += R * TL) tw[e] = twiddle<T>(a.ax, e);
+    const long nlines = a.nlines;
+    const long pair = (long)blockIdx.x * TL + l;
+    const long la = 2 * pair, lb = la + 1;
+    const bool va = la < nlines, vb = lb < nlines;
+    T* M = lds + l * MS;
+    C v[R];
+    if (a.mode == XMODE_R2C) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int x = c + R * j;
+            v[j] = cmk<C>(va ? a.rin[la * N + x] : (T)0, vb ? a.rin[lb * N + x] : (T)0);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int k = c + R * j;
+            const int kk = k < NH ? k : N - k;
+            C A = va ? a.cin[la * a.ldh + kk] : cmk<C>((T)0, (T)0);
+            C B = vb ? a.cin[lb * a.ldh + kk] : cmk<C>((T)0, (T)0);
+            if (kk == 0 || 2 * kk == N) { A.y = (T)0; B.y = (T)0; }
+            if (k != kk) { A.y = -A.y; B.y = -B.y; }
+            v[j] = cmk<C>(A.x - B.y, A.y + B.x);
+        }
+    }
+    __syncthreads();
+    const bool inv = a.mode != XMODE_R2C;
+    auto to_rows = [=](int k) { return k * RS + c; };
+    auto from_row = [=](int n) { return c * RS + n; };
+    rdft<C, R>(v, inv);
+    twiddle_regs<C, R>(v, c, tw, inv);
+    xchg_split<C, R, R>(v, v, M, to_rows, from_row);
+    rdft<C, R>(v, inv);  // v[k2] = Z[c + R k2]
+    if (inv) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int x = c + R * k;
+            const long ia = la * N + x, ib = lb * N + x;
+            const T ra = v[k].x * a.scale, rb = v[k].y * a.scale;
+            if (a.mode == XMODE_C2R) {
+                if (va) a.rout[ia] = ra;
+                if (vb) a.rout[ib] = rb;
+            } else if (a.mode == XMODE_CG_S) {
+                const T sa = va ? cgs_point<T>(a, ia, ra) : (T)0;
+                const T sb = vb ? cgs_point<T>(a, ib, rb) : (T)0;
+                v[k] = cmk<C>(sa, sb);
+            } else {  // POT
+                if (va) pot_point<T>(a, ia, ra);
+                if (vb) pot_point<T>(a, ib, rb);
+            }
+        }
+        if (a.mode != XMODE_CG_S) return;
+        // forward transform of s held by rows (x = c + R k): into columns
+        rdft<C, R>(v, false);
+        twiddle_regs<C, R>(v, c, tw, false);
+        xchg_split<C, R, R>(v, v, M, to_rows, from_row);
+        rdft<C, R>(v, false);  // v[m] = S[c + R m]
+    }
+    // natural order, then the (k, N-k) pairs of the split
+    C z[2 * NJ];
+    xchg_split<C, R, 2 * NJ>(
+        v, z, M, [=](int m) { const int p = c + R * m; return p + p / R; },
+        [=](int n) {
+            const int k = c + R * (n >> 1);
+            const int q = (n & 1) ? ((N - k) & (N - 1)) : k;
+            return q + q / R;
+        });
+    const T h = (T)0.5;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int k = c + R * j;
+        if (k < NH) {
+            const C zk = z[2 * j], zn = z[2 * j + 1];
+            if (va) a.cout[la * a.ldh + k] = cmk<C>(h * (zk.x + zn.x), h * (zk.y - zn.y));
+            if (vb) a.cout[lb * a.ldh + k] = cmk<C>(h * (zk.y + zn.y), -h * (zk.x - zn.x));
+        }
+    }
+}
+
 // ------------------------------------------------------------------ y pass
 // Block: TX consecutive kx columns of one z plane (tid = c*TX + l).
 
@@ -338,6 +461,15 @@ template <typename T>
 hipError_t launch_xreg(XPassArgs<T> a, hipStream_t st) {
     const int R = reg_radix(a.ax.n);
     const long npairs = (a.nlines + 1) / 2;
+    if (a.split) {
+        switch (R) {
+        case 4: hipLaunchKernelGGL((k_xreg_s<T, 4, 32>), dim3((npairs + 31) / 32), dim3(128), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_xreg_s<T, 8, 16>), dim3((npairs + 15) / 16), dim3(128), 0, st, a); break;
+        case 16: hipLaunchKernelGGL((k_xreg_s<T, 16, 8>), dim3((npairs + 7) / 8), dim3(128), 0, st, a); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (R) {
     case 4: hipLaunchKernelGGL((k_xreg<T, 4, 32>), dim3((npairs + 31) / 32), dim3(128), 0, st, a); break;
     case 8: hipLaunchKernelGGL((k_xreg<T, 8, 16>), dim3((npairs + 15) / 16), dim3(128), 0, st, a); break;

@@ -16,6 +16,7 @@ template <typename T> struct XPassArgs {
     int ldh;      // row stride of the half spectra (>= nx/2+1, padded for aligned rows)
     int ppb;      // line pairs per block (set by the launcher)
     int mode;
+    int split;    // register passes: exchange real and imaginary parts in turn (half the LDS)
     T scale;      // applied after an inverse transform (renor)
     const T* rin;        // R2C: input; CG_S: den; POT: den
     const T* rin2;       // POT: dencg

@@ -23,16 +23,18 @@ def say(*a):
     sys.stdout.flush()
 
 
-def make(shape, prec, icon, graphs, n4=150, seed=3, r0=(0.3, -0.2, 0.1), v0=(1.0, 2.0, -3.0)):
+def make(shape, prec, icon, graphs, n4=150, seed=3, r0=(0.3, -0.2, 0.1), v0=(1.0, 2.0, -3.0), xsplit=False):
     nz, ny, nx = shape
     h = 0.6
     p = capi.default_params(nx=nx, ny=ny, nz=nz, xmax=nx * h / 2, ymax=ny * h / 2, zmax=nz * h / 2, npd=13,
                             icon=icon, precision=prec)
     os.environ["HETD_GRAPHS"] = "1" if graphs else "0"
+    os.environ["HETD_XSPLIT"] = "1" if xsplit else "0"
     try:
         ctx = Context(p, selec="Xe_He", r_cutoff=2.55, umax=2793.9365463)
     finally:
         del os.environ["HETD_GRAPHS"]
+        del os.environ["HETD_XSPLIT"]
     ctx.set_state(woods_saxon_state(nx, ny, nz, p.xmax, p.ymax, p.zmax, n4, seed), r0, v0)
     ctx.init()
     return ctx
@@ -98,5 +100,23 @@ def test_history_restart_is_bitwise(shape, prec, icon):
         same_state(a, b)
     with pytest.raises(RuntimeError):
         b.load_history(blob[:-8])
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("shape,prec,icon", [((20, 18, 256), capi.HETD_FP64, 13), ((16, 24, 64), capi.HETD_MIXED, 8),
+                                             ((30, 20, 16), capi.HETD_FP64, 13), ((12, 10, 256), capi.HETD_MIXED, 13)])
+def test_split_exchange_x_pass_is_bitwise(shape, prec, icon):
+    """The x passes with the LDS exchange split in real and imaginary halves
+    (HETD_XSPLIT) compute the same bits as the complex exchange."""
+    say("xsplit", shape, prec, icon)
+    a = make(shape, prec, icon, graphs=True)
+    b = make(shape, prec, icon, graphs=True, xsplit=True)
+    for n in (5, 9):
+        a.evolve(G.DT, n, traj=False)
+        b.evolve(G.DT, n, traj=False)
+        same_state(a, b)
+    for w in (capi.F_POT4, capi.F_DENCG, capi.F_DELJ4):
+        assert np.array_equal(a.field(w), b.field(w))
     a.close()
     b.close()
