@@ -342,10 +342,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "mixed: f32 density/mean field/FFT, f64 state and reductions" if head == "mixed" else "f64",
+        "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic (Woods-Saxon He-1000 droplet, seeded phase, Xe at the centre moving 200 m/s)",
         "config": {"workload": "He-1000+Xe %d^3 0.4A OT+solid npd13 icon13 dt0.5fs predictor-corrector steps" % n,
-                   "grid": [n, n, n], "precision": head, "wall_ms_per_step": round(r["wall"] * 1e3 / args.steps, 4),
+                   "grid": [n, n, n], "precision": head,
+                   "precision_detail": ("f32 density, mean field and FFTs; f64 wave function, integrator "
+                                        "state and reductions" if head == "mixed" else "f64 throughout"),
+                   "wall_ms_per_step": round(r["wall"] * 1e3 / args.steps, 4),
                    "per_precision": per_prec, "mixed_vs_fp64": cmp_, "ensemble": ens},
         "roofline": {"bound": "hbm", "kernel": args.kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4),
