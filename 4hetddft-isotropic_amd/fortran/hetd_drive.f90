@@ -319,29 +319,32 @@ contains
   subroutine read_mode0(fname)
     ! readenc mode 0 (readenc.f90:80-100): a density, or else a complex
     ! wave function, after a grid line that also carries limp and an
-    ! impurity position (read and not used: the namelist's stays).  A
-    ! density becomes psi = sqrt(den) exp(i k.r), the boost of
-    ! DFT4He3d.f90:606-632.  The reference forms the phase with cmplx()
-    ! of default kind, so its cosine and sine are rounded to REAL(4);
-    ! the same rounding is kept here.
+    ! impurity position (read and not used: the namelist's stays).  The
+    ! reference tells them apart by the error a real read of "(re,im)" gives;
+    ! that error depends on the compiler's runtime, so here the first value
+    ! is looked at: a complex one opens with "(".  A density becomes
+    ! psi = sqrt(den) exp(i k.r), the boost of DFT4He3d.f90:606-632.  The
+    ! reference forms the phase with cmplx() of default kind, so its cosine
+    ! and sine are rounded to REAL(4); the same rounding is kept here.
     character(len=*), intent(in) :: fname
     real(8) :: xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, xf, yf, zf, ph, xx, yy, zz
     real(8), allocatable :: den(:,:,:)
-    integer :: nxp, nyp, nzp, ios, ix, iy, iz
+    integer :: nxp, nyp, nzp, ix, iy, iz
     logical :: limpf
+    character(len=64) :: head
     allocate(den(nx,ny,nz))
     open(1, file=fname, status='old')
     call titols(1)
     read(1, *) xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, nxp, nyp, nzp, limpf, xf, yf, zf
     call check_grid(nxp, nyp, nzp)
-    read(1, *, iostat=ios) den
-    if (ios /= 0) then
-      rewind(1)
-      call titols(1)
-      read(1, *) xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, nxp, nyp, nzp, limpf, xf, yf, zf
+    read(1, '(A)') head
+    backspace(1)
+    head = adjustl(head)
+    if (head(1:1) == '(') then
       read(1, *) psi
       write(6, '("From Readenc: We have read a complex w.f.")')
     else
+      read(1, *) den
       write(6, '("From Readenc: We have read a density")')
       if (xlamda /= 0.d0) xlamdaz = xlamda*ktops/(2.d0*h2o2m4)
       do iz = 1, nz

@@ -137,8 +137,9 @@ def test_driver_mode0_density_boost_matches_mode2(tmp_path):
     write_mode2(os.path.join(m2, "den.in"), psi, cfg["xmax"], cfg["ymax"], cfg["zmax"], r0, v0)
     vps = v0 / 7.63822291
     common = dict(niter=6, xlamdax=repr(kx), xlamday=repr(ky), xlamda=repr(lam))
-    a = run_drive(m0, namelist(cfg, mode=0, ximp=repr(r0[0]), yimp=repr(r0[1]), zimp=repr(r0[2]),
-                               vximp=repr(vps[0]), vyimp=repr(vps[1]), vzimp=repr(vps[2]), **common))
+    a = run_drive(m0, namelist(cfg, mode=0, ximp=repr(float(r0[0])), yimp=repr(float(r0[1])),
+                               zimp=repr(float(r0[2])), vximp=repr(float(vps[0])), vyimp=repr(float(vps[1])),
+                               vzimp=repr(float(vps[2])), **common))
     assert a.returncode == 0, a.stderr
     assert "We have read a density" in a.stdout
     b = run_drive(m2, namelist(cfg, **common))
@@ -175,8 +176,8 @@ def test_driver_mode0_wavefunction_and_mode3(tmp_path):
     for m, w in dirs.items():
         over = dict(niter=5, mode=int(m[1]))
         if m == "m0":
-            over.update(ximp=repr(r0[0]), yimp=repr(r0[1]), zimp=repr(r0[2]),
-                        vximp=repr(vps[0]), vyimp=repr(vps[1]), vzimp=repr(vps[2]))
+            over.update(ximp=repr(float(r0[0])), yimp=repr(float(r0[1])), zimp=repr(float(r0[2])),
+                        vximp=repr(float(vps[0])), vyimp=repr(float(vps[1])), vzimp=repr(float(vps[2])))
         r = run_drive(w, namelist(cfg, **over))
         assert r.returncode == 0, r.stderr
         out[m] = (np.loadtxt(os.path.join(w, "rimp.out"), comments="#"), read_mode2(os.path.join(w, "den.out"))[0])
