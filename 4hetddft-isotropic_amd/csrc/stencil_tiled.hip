@@ -98,15 +98,17 @@ template <bool MIX> __device__ __forceinline__ void put_den(const UpdateArgs& a,
 // forceimp (potenimp.f90:24-55) of the state the update starts from.
 // MIX (UPDATE only): FP32 mean field in, FP32 den and complex64 RHS history.
 // SK (UPDATE only): the stage, STAGE_RK / STAGE_PRED / STAGE_CORR.
-template <int KIND, int M, int TPY, int TNT, bool PF, bool FRC, bool MIX, int SK>
+template <int KIND, int M, int TPY, int TNT, bool PF, bool FRC, bool MIX, int SK, bool PP>
 __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KIND>::A args) {
     using V = typename TArgs<KIND>::V;
     constexpr int NC = TArgs<KIND>::NC;
     constexpr int RS = TNT / TTX;        // thread rows
     constexpr int TTY = TPY * RS;
     constexpr int LX = TTX + 2 * M, LY = TTY + 2 * M, W = 2 * M + 1;
-    __shared__ V lds[LX * LY];
-    __shared__ double ldsd[FRC ? LX * LY : 1];
+    // PP: two LDS planes used in turn, so a plane is written while the
+    // previous one may still be read, and one barrier per plane suffices
+    __shared__ V lds_pl[PP ? 2 : 1][LX * LY];
+    __shared__ double ldsd_pl[PP ? 2 : 1][FRC ? LX * LY : 1];
     __shared__ double red[NC * TNT / 64];
 
     const Grid& g = tc.g;
@@ -276,13 +278,18 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
     }
 
     for (int iz = iz0; iz < izend; ++iz) {
+        // with PP, plane iz goes to buffer iz & 1: the barrier of plane iz - 1
+        // follows every read of plane iz - 2, the last user of that buffer
+        V* lds = lds_pl[PP ? (iz & 1) : 0];
+        double* ldsd = ldsd_pl[PP ? (iz & 1) : 0];
+        (void)ldsd;
 #pragma unroll
         for (int p = 0; p < TPY; ++p) {
             const V qn = PF ? vz<V>::mode(qraw[p], qm[p]) : zcol(p, iz + M);
             q[p][W - 1] = qn;
             if constexpr (FRC) dq[p][W - 1] = dens(qn);
         }
-        __syncthreads();
+        if constexpr (!PP) __syncthreads();
         // plane iz into LDS: interior from the queues, halo from the prefetch
 #pragma unroll
         for (int p = 0; p < TPY; ++p) {
@@ -522,24 +529,27 @@ static TiledCommon tiled_common(const Grid& g, const StencilCoef& sc, int tty) {
 
 bool tiled_supported(const StencilCoef& sc) { return sc.m == 4 || sc.m == 6; }
 
-// tile variants (points per thread, threads, pipelined loads).  Measured on
-// MI355X at 256^3: one point per thread wins for every kind (the complex
-// register queue of two points caps a SIMD at 1-2 waves); for UPDATE, 256
-// threads (64x4 tiles, several workgroups per CU, so one workgroup's loads
-// overlap another's arithmetic) with the next plane's loads issued before
-// the current plane is computed beat 512-thread 64x8 tiles by 0.15 ms/step.
-// HETD_TILE_FORCE / HETD_TILE_ENERGY = 0..3 select another variant for those
-// kinds; the update kernels (one per stage) are built for variant 3 only.
-struct TileVariant { int tpy, tnt; bool pf; };
-static constexpr int kNVariants = 4;
-static constexpr TileVariant kVariants[kNVariants] = {{1, 512, false}, {1, 256, false}, {1, 512, true}, {1, 256, true}};
+// tile variants (points per thread, threads, pipelined loads, ping-pong LDS
+// planes).  Measured on MI355X at 256^3: one point per thread wins for every
+// kind (the complex register queue of two points caps a SIMD at 1-2 waves);
+// for UPDATE, 256 threads (64x4 tiles, several workgroups per CU, so one
+// workgroup's loads overlap another's arithmetic) with the next plane's loads
+// issued before the current plane is computed beat 512-thread 64x8 tiles by
+// 0.15 ms/step.  HETD_TILE_FORCE / HETD_TILE_ENERGY = 0..4 select another
+// variant for those kinds; the update kernels (one per stage) are built for
+// variant 3 and, selected by HETD_TILE_UPDATE=4, variant 4 (variant 3 with
+// ping-pong planes: one barrier per plane instead of two).
+struct TileVariant { int tpy, tnt; bool pf, pp; };
+static constexpr int kNVariants = 5;
+static constexpr TileVariant kVariants[kNVariants] = {
+    {1, 512, false, false}, {1, 256, false, false}, {1, 512, true, false}, {1, 256, true, false}, {1, 256, true, true}};
 
-static constexpr int kUpdateVariant = 3;  // the update kernels are built for this one only
+static constexpr int kUpdateVariant = 3;  // the default update kernels
 
 static int variant_for(int kind) {
-    if (kind == TK_UPDATE) return kUpdateVariant;
-    static const char* names[3] = {"", "HETD_TILE_FORCE", "HETD_TILE_ENERGY"};
+    static const char* names[3] = {"HETD_TILE_UPDATE", "HETD_TILE_FORCE", "HETD_TILE_ENERGY"};
     const char* e = getenv(names[kind]);
+    if (kind == TK_UPDATE) return e && e[0] == '4' ? 4 : kUpdateVariant;
     if (e && e[0] >= '0' && e[0] < '0' + kNVariants) return e[0] - '0';
     return 0;
 }
@@ -547,8 +557,8 @@ static int variant_for(int kind) {
 template <int KIND, int M, int V, bool FRC, bool MIX, int SK>
 static void launch_k(const TiledCommon& tc, const typename TArgs<KIND>::A& a, int nlaunch, hipStream_t st) {
     constexpr TileVariant tv = kVariants[V];
-    hipLaunchKernelGGL((k_tiled<KIND, M, tv.tpy, tv.tnt, tv.pf, FRC, MIX, SK>), dim3(nlaunch), dim3(tv.tnt), 0, st,
-                       tc, a);
+    hipLaunchKernelGGL((k_tiled<KIND, M, tv.tpy, tv.tnt, tv.pf, FRC, MIX, SK, tv.pp>), dim3(nlaunch), dim3(tv.tnt),
+                       0, st, tc, a);
 }
 
 template <int M, int V, bool FRC, bool MIX>
@@ -579,13 +589,15 @@ static void launch_v(const TiledCommon& tc, const typename TArgs<KIND>::A& a, in
 template <int KIND, int M>
 static void launch_m(int v, const TiledCommon& tc, const typename TArgs<KIND>::A& a, int nlaunch, hipStream_t st) {
     if constexpr (KIND == TK_UPDATE) {
-        launch_v<KIND, M, kUpdateVariant>(tc, a, nlaunch, st);
+        if (v == 4) launch_v<KIND, M, 4>(tc, a, nlaunch, st);
+        else launch_v<KIND, M, kUpdateVariant>(tc, a, nlaunch, st);
     } else {
         switch (v) {
         case 0: launch_v<KIND, M, 0>(tc, a, nlaunch, st); break;
         case 1: launch_v<KIND, M, 1>(tc, a, nlaunch, st); break;
         case 2: launch_v<KIND, M, 2>(tc, a, nlaunch, st); break;
-        default: launch_v<KIND, M, 3>(tc, a, nlaunch, st); break;
+        case 3: launch_v<KIND, M, 3>(tc, a, nlaunch, st); break;
+        default: launch_v<KIND, M, 4>(tc, a, nlaunch, st); break;
         }
     }
 }

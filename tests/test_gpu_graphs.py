@@ -120,3 +120,23 @@ def test_split_exchange_x_pass_is_bitwise(shape, prec, icon):
         assert np.array_equal(a.field(w), b.field(w))
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("shape,prec,icon", [((36, 40, 48), capi.HETD_FP64, 13), ((32, 24, 40), capi.HETD_MIXED, 8),
+                                             ((30, 32, 45), capi.HETD_FP64, 18)])
+def test_pingpong_update_variant_is_bitwise(shape, prec, icon):
+    """HETD_TILE_UPDATE=4 (update kernels with two LDS planes in turn, one
+    barrier per plane) computes the same bits as the default update kernels."""
+    say("pingpong", shape, prec, icon)
+    a = make(shape, prec, icon, graphs=False)
+    b = make(shape, prec, icon, graphs=False)
+    for n in (5, 8):
+        a.evolve(G.DT, n, traj=False)
+        os.environ["HETD_TILE_UPDATE"] = "4"
+        try:
+            b.evolve(G.DT, n, traj=False)
+        finally:
+            del os.environ["HETD_TILE_UPDATE"]
+        same_state(a, b)
+    a.close()
+    b.close()
