@@ -75,7 +75,7 @@ struct hetd_ctx {
     bool have_state = false;
     // predictor-corrector steps replayed from a HIP graph (see step_graph)
     bool use_graphs = true;
-    bool xsplit = false;  // x passes with split LDS exchanges (HETD_XSPLIT)
+    bool xsplit = false;  // x and y register passes with split LDS exchanges (HETD_XSPLIT)
     hipGraphExec_t gexec = nullptr;
     int gper = 0;        // steps per replay (the period of the buffer rotation)
     double gdt = 0.0;
@@ -172,6 +172,7 @@ template <typename T> static hipError_t run_x(hetd_ctx* c, XPassArgs<T> x) {
 
 template <typename T> static hipError_t run_y(hetd_ctx* c, YPassArgs<T> y, int nbuf) {
     y.ldh = c->ldh;
+    y.split = c->xsplit ? 1 : 0;
     tbeg(c, HETD_K_YPASS);
     hipError_t e = reg_fft_supported(y.ax.n) ? launch_yreg(y, c->nz, nbuf, c->st)
                    : reg2_supported(y.ax.n)  ? launch_y2(y, c->nz, nbuf, c->st)

@@ -360,6 +360,34 @@ __global__ void __launch_bounds__(R* TX) k_yreg(YPassArgs<T> a) {
         for (int k = 0; k < R; ++k) buf[base + (long)ldh * (c + R * k)] = v[k];
 }
 
+// y pass with the split exchange of k_xreg_s (same arithmetic as k_yreg)
+template <typename T, int R, int TX>
+__global__ void __launch_bounds__(R* TX) k_yreg_s(YPassArgs<T> a) {
+    using C = cplx<T>;
+    constexpr int N = R * R;
+    __shared__ T lds[N * TX];
+    __shared__ C tw[N];
+    const int tid = threadIdx.x, l = tid % TX, c = tid / TX;
+    for (int e = tid; e < N; e += R * TX) tw[e] = twiddle<T>(a.ax, e);
+    const int nxh = a.nxh, ldh = a.ldh, kx = blockIdx.x * TX + l, z = blockIdx.y;
+    C* buf = a.buf[blockIdx.z];
+    const long base = (long)ldh * N * z + kx;
+    const bool ok = kx < nxh;
+    const bool inv = a.inv != 0;
+    C v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = ok ? buf[base + (long)ldh * (c + R * j)] : cmk<C>((T)0, (T)0);
+    __syncthreads();
+    rdft<C, R>(v, inv);
+    twiddle_regs<C, R>(v, c, tw, inv);
+    xchg_split<C, R, R>(v, v, lds, [=](int k) { return (k * R + c) * TX + l; },
+                        [=](int n) { return (c * R + n) * TX + l; });
+    rdft<C, R>(v, inv);
+    if (ok)
+#pragma unroll
+        for (int k = 0; k < R; ++k) buf[base + (long)ldh * (c + R * k)] = v[k];
+}
+
 // ------------------------------------------------------------------ z pass
 
 // NK1: at most one kernel (FWD, INV, or CONV with nk = 1): the product is
@@ -482,6 +510,17 @@ hipError_t launch_xreg(XPassArgs<T> a, hipStream_t st) {
 template <typename T>
 hipError_t launch_yreg(YPassArgs<T> a, int nz, int nbuf, hipStream_t st) {
     const int R = reg_radix(a.ax.n);
+    if (a.split) {
+        switch (R) {
+        case 4: hipLaunchKernelGGL((k_yreg_s<T, 4, 32>), dim3((a.nxh + 31) / 32, nz, nbuf), dim3(128), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_yreg_s<T, 8, 16>), dim3((a.nxh + 15) / 16, nz, nbuf), dim3(128), 0, st, a); break;
+        case 16:
+            hipLaunchKernelGGL((k_yreg_s<T, 16, 16>), dim3((a.nxh + 15) / 16, nz, nbuf), dim3(256), 0, st, a);
+            break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (R) {
     case 4: hipLaunchKernelGGL((k_yreg<T, 4, 32>), dim3((a.nxh + 31) / 32, nz, nbuf), dim3(128), 0, st, a); break;
     case 8: hipLaunchKernelGGL((k_yreg<T, 8, 16>), dim3((a.nxh + 15) / 16, nz, nbuf), dim3(128), 0, st, a); break;

@@ -67,6 +67,7 @@ template <typename T> __device__ __forceinline__ void pot_point(const XPassArgs<
 template <typename T> struct YPassArgs {
     FftAxis ax;
     int nxh, ldh, tx, inv;  // kx count, row stride
+    int split;              // register passes: exchange real and imaginary parts in turn (half the LDS)
     cplx<T>* buf[4];
 };
 

@@ -105,10 +105,12 @@ def test_history_restart_is_bitwise(shape, prec, icon):
 
 
 @pytest.mark.parametrize("shape,prec,icon", [((20, 18, 256), capi.HETD_FP64, 13), ((16, 24, 64), capi.HETD_MIXED, 8),
-                                             ((30, 20, 16), capi.HETD_FP64, 13), ((24, 24, 256), capi.HETD_MIXED, 13)])
+                                             ((30, 20, 16), capi.HETD_FP64, 13), ((24, 24, 256), capi.HETD_MIXED, 13),
+                                             ((32, 64, 36), capi.HETD_FP64, 13), ((30, 16, 40), capi.HETD_MIXED, 8)])
 def test_split_exchange_x_pass_is_bitwise(shape, prec, icon):
-    """The x passes with the LDS exchange split in real and imaginary halves
-    (HETD_XSPLIT) compute the same bits as the complex exchange."""
+    """The x and y register passes with the LDS exchange split in real and
+    imaginary halves (HETD_XSPLIT) compute the same bits as the complex
+    exchange (x lengths 256, 64, 16; y lengths 64, 16)."""
     say("xsplit", shape, prec, icon)
     a = make(shape, prec, icon, graphs=True)
     b = make(shape, prec, icon, graphs=True, xsplit=True)
