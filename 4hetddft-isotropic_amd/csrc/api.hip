@@ -26,7 +26,7 @@ struct hetd_ctx {
     bool otc = false;
     hipStream_t st = nullptr;
     std::string err;
-    std::vector<void*> allocs;
+    std::vector<std::pair<void*, size_t>> allocs;  // every device buffer and its size
     // grid tables
     double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_tcx = nullptr, *d_tcy = nullptr, *d_tcz = nullptr;
     double2* d_tw[3] = {nullptr, nullptr, nullptr};
@@ -76,6 +76,7 @@ struct hetd_ctx {
     // predictor-corrector steps replayed from a HIP graph (see step_graph)
     bool use_graphs = true;
     bool xsplit = false;  // x and y register passes with split LDS exchanges (HETD_XSPLIT)
+    int upd_tile = 0;     // update-kernel tile variant (0: default / HETD_TILE_UPDATE), set by the autotuner
     hipGraphExec_t gexec = nullptr;
     int gper = 0;        // steps per replay (the period of the buffer rotation)
     double gdt = 0.0;
@@ -110,7 +111,7 @@ template <typename P> static hipError_t dalloc(hetd_ctx* c, P** p, size_t bytes)
     void* v = nullptr;
     hipError_t e = hipMalloc(&v, bytes ? bytes : 16);
     if (e == hipSuccess) {
-        c->allocs.push_back(v);
+        c->allocs.push_back({v, bytes ? bytes : 16});
         *p = reinterpret_cast<P*>(v);
     }
     return e;
@@ -555,6 +556,7 @@ static UpdateArgs update_base(hetd_ctx* c, double dt) {
     u.g = c->grid; u.sc = c->sc; u.h2o2m4 = c->p.h2o2m4; u.dt = dt; u.psi = c->psi; u.pot4 = c->pot4;
     u.imp = c->imp; u.den = c->den; u.guard = c->guard; u.partial = c->partial;
     if (c->mixed) { u.pot4f = c->pot4_32; u.denf = c->den32; }
+    u.tile = c->upd_tile;
     return u;
 }
 
@@ -1010,7 +1012,7 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
 int hetd_finalize(hetd_ctx* c) {
     if (!c) return HETD_ERR_ARG;
     if (c->st) (void)hipStreamSynchronize(c->st);
-    for (void* v : c->allocs) (void)hipFree(v);
+    for (auto& a : c->allocs) (void)hipFree(a.first);
     if (c->traj) (void)hipFree(c->traj);
     if (c->dtmp[0]) (void)hipFree(c->dtmp[0]);
     if (c->dtmp[1]) (void)hipFree(c->dtmp[1]);
@@ -1292,6 +1294,93 @@ int hetd_pool_evolve(hetd_ctx* const* ctxs, int32_t n, double dt, int32_t nsteps
         if (int rc = settle_force(ctxs[i])) return rc;
     for (int i = 0; i < n; ++i)
         if (int rc = check_guard(ctxs[i])) return rc;
+    return HETD_OK;
+}
+
+// ------------------------------------------------------------ autotuner
+//
+// Chooses, by timing on the context's own grid and state, among kernel
+// variants that compute the same bits, so that tuning never changes a result:
+// the x/y register passes with a complex or a split LDS exchange, and the
+// update kernels with one or two LDS planes.  Every device buffer is copied
+// aside first and copied back after each trial, with the host-side step state.
+
+int hetd_autotune(hetd_ctx* c, double dt, int32_t steps, int32_t* choice) {
+    if (!c || steps < 1 || !(dt > 0.0)) return HETD_ERR_ARG;
+    if (!c->have_state) return fail(c, HETD_ERR_STATE, "hetd_set_state must precede hetd_autotune");
+    if (c->tclass) return fail(c, HETD_ERR_STATE, "hetd_autotune with kernel-class timing on");
+    HCHK(c, hipStreamSynchronize(c->st));
+    std::vector<void*> snap;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(c->st);
+        for (void* p : snap) (void)hipFree(p);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
+    const size_t nsnap = c->allocs.size();
+    for (size_t i = 0; i < nsnap; ++i) {
+        void* p = nullptr;
+        if (hipMalloc(&p, c->allocs[i].second) != hipSuccess) {
+            cleanup();
+            return fail(c, HETD_ERR_HIP, "hetd_autotune: no device memory for the state copy");
+        }
+        snap.push_back(p);
+        if (hipMemcpyAsync(p, c->allocs[i].first, c->allocs[i].second, hipMemcpyDeviceToDevice, c->st) != hipSuccess) {
+            cleanup();
+            return fail(c, HETD_ERR_HIP, "hetd_autotune: state copy failed");
+        }
+    }
+    const std::vector<long> sig = step_sig(c);
+    const long iters = c->iter_since_set;
+    double* const prec = c->pend_rec;
+    const bool xs0 = c->xsplit;
+    const int ut0 = c->upd_tile;
+    auto restore = [&]() -> int {
+        for (size_t i = 0; i < nsnap; ++i)
+            HCHK(c, hipMemcpyAsync(c->allocs[i].first, snap[i], c->allocs[i].second, hipMemcpyDeviceToDevice, c->st));
+        restore_sig(c, sig);
+        c->iter_since_set = iters;
+        c->pend_rec = prec;
+        HCHK(c, hipStreamSynchronize(c->st));
+        return 0;
+    };
+    auto give_up = [&](int rc) {
+        c->xsplit = xs0;
+        c->upd_tile = ut0;
+        drop_graph(c);
+        (void)restore();
+        cleanup();
+        return rc;
+    };
+    for (hipEvent_t& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return give_up(fail(c, HETD_ERR_HIP, "hetd_autotune: events"));
+    float best = 0.0f;
+    int bestv = 0;
+    for (int v = 0; v < 4; ++v) {
+        c->xsplit = (v & 1) != 0;
+        c->upd_tile = (v & 2) ? 4 : 0;
+        drop_graph(c);
+        // an untimed pass first: it builds this variant's step graph
+        if (int rc = hetd_evolve(c, dt, steps, nullptr)) return give_up(rc);
+        if (int rc = restore()) return give_up(rc);
+        if (hipEventRecord(ev[0], c->st) != hipSuccess) return give_up(fail(c, HETD_ERR_HIP, "event"));
+        if (int rc = hetd_evolve(c, dt, steps, nullptr)) return give_up(rc);
+        if (hipEventRecord(ev[1], c->st) != hipSuccess) return give_up(fail(c, HETD_ERR_HIP, "event"));
+        if (hipStreamSynchronize(c->st) != hipSuccess) return give_up(fail(c, HETD_ERR_HIP, "sync"));
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) return give_up(fail(c, HETD_ERR_HIP, "timing"));
+        if (int rc = restore()) return give_up(rc);
+        if (v == 0 || ms < best) {
+            best = ms;
+            bestv = v;
+        }
+    }
+    c->xsplit = (bestv & 1) != 0;
+    c->upd_tile = (bestv & 2) ? 4 : 0;
+    drop_graph(c);
+    cleanup();
+    if (choice) *choice = bestv;
     return HETD_OK;
 }
 

@@ -133,6 +133,7 @@ struct UpdateArgs {
     StencilCoef sc;
     double h2o2m4;
     int kind;
+    int tile;  // tile variant of the tiled update kernels (0: the default or HETD_TILE_UPDATE)
     int rk_j;  // RK stage 0..3
     double dt;
     double ra, rb, rc;  // Gill coefficients of the stage

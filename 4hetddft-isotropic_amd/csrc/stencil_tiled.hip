@@ -604,7 +604,9 @@ static void launch_m(int v, const TiledCommon& tc, const typename TArgs<KIND>::A
 
 template <int KIND>
 static hipError_t launch_tiled(const typename TArgs<KIND>::A& a, hipStream_t st, int* nblocks) {
-    const int v = variant_for(KIND);
+    int v = variant_for(KIND);
+    if constexpr (KIND == TK_UPDATE)
+        if (a.tile == 4 || a.tile == kUpdateVariant) v = a.tile;
     const TileVariant tv = kVariants[v];
     TiledCommon tc = tiled_common(a.g, a.sc, tv.tpy * tv.tnt / TTX);
     const int nb = tc.ntx * tc.nty * tc.ntz;

@@ -103,6 +103,13 @@ module hetd_capi
       integer(c_int32_t), value :: lsurf3d
       real(c_double), value :: zsurf
     end function
+    integer(c_int) function hetd_autotune(ctx, dt, steps, choice) bind(C, name='hetd_autotune')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt
+      integer(c_int32_t), value :: steps
+      integer(c_int32_t), intent(out) :: choice
+    end function
     integer(c_int) function hetd_steprk(ctx, dt) bind(C, name='hetd_steprk')
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx

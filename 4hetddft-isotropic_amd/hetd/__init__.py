@@ -166,6 +166,13 @@ class Context:
         self._chk(self.L.hetd_evolve(self.h, dt, nsteps, _p(t) if traj else None))
         return t[:nsteps]
 
+    def autotune(self, dt, steps):
+        """Pick the fastest of the kernel variants that give the same bits
+        (hetd_autotune); the state is unchanged.  Returns the choice bits."""
+        ch = C.c_int32()
+        self._chk(self.L.hetd_autotune(self.h, dt, steps, C.byref(ch)))
+        return ch.value
+
     def field(self, which):
         if which in capi.COMPLEX_FIELDS:
             out = np.zeros(self.shape, dtype=np.complex128)

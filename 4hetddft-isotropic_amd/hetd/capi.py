@@ -124,6 +124,7 @@ def _proto(lib):
     lib.hetd_save_history.argtypes = [_vp, _vp]
     lib.hetd_load_history.argtypes = [_vp, _vp, C.c_int64]
     lib.hetd_evolve.argtypes = [_vp, C.c_double, C.c_int32, _dp]
+    lib.hetd_autotune.argtypes = [_vp, C.c_double, C.c_int32, C.POINTER(C.c_int32)]
     lib.hetd_pool_evolve.argtypes = [C.POINTER(_vp), C.c_int32, C.c_double, C.c_int32]
     lib.hetd_get_field.argtypes = [_vp, C.c_int32, _dp]
     lib.hetd_get_imp.argtypes = [_vp, _dp, _dp, _dp]
@@ -141,7 +142,7 @@ def _proto(lib):
                  "hetd_set_potion", "hetd_set_state", "hetd_potenimp", "hetd_poten",
                  "hetd_forceimp", "hetd_set_aimp", "hetd_steprk", "hetd_steppc",
                  "hetd_energy", "hetd_norm", "hetd_moments", "hetd_save_history", "hetd_load_history",
-                 "hetd_evolve", "hetd_pool_evolve", "hetd_get_field",
+                 "hetd_evolve", "hetd_pool_evolve", "hetd_autotune", "hetd_get_field",
                  "hetd_get_imp", "hetd_fft_r2c", "hetd_fft_c2r", "hetd_deriv"):
         getattr(lib, name).restype = C.c_int
     return lib
@@ -152,7 +153,7 @@ EXPORTED = ("hetd_default_params", "hetd_init", "hetd_finalize", "hetd_last_erro
             "hetd_set_surface", "hetd_set_state",
             "hetd_potenimp", "hetd_poten", "hetd_forceimp", "hetd_set_aimp",
             "hetd_steprk", "hetd_steppc", "hetd_energy", "hetd_norm", "hetd_moments", "hetd_evolve",
-            "hetd_history_size", "hetd_save_history", "hetd_load_history", "hetd_pool_evolve",
+            "hetd_history_size", "hetd_save_history", "hetd_load_history", "hetd_pool_evolve", "hetd_autotune",
             "hetd_get_field", "hetd_get_imp", "hetd_fft_r2c", "hetd_fft_c2r",
             "hetd_deriv", "hetd_stream", "hetd_timing", "hetd_timing_read")
 

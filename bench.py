@@ -145,6 +145,7 @@ def run_precision(args, prec, psi, kclass, ev):
     ctx.init()
     dt = 5e-4 / KTOPS
     ctx.evolve(dt, max(args.warmup, 3), traj=False)
+    tuned = ctx.autotune(dt, args.autotune) if args.autotune > 0 else None
     e0 = ctx.energy()
     setup_s = time.perf_counter() - t0
     stream = ctx.stream()
@@ -171,7 +172,7 @@ def run_precision(args, prec, psi, kclass, ev):
     ctx.close()
     nsteps_total = max(args.warmup, 3) + args.steps + 2 * len(capi.K_NAMES)
     return {"ms": ms, "wall": wall, "kms": kms, "klaunch": klaunch, "kbytes": kbytes, "classes": classes,
-            "e0": e0, "e1": e1, "norm": norm, "setup_s": setup_s, "nsteps_total": nsteps_total}
+            "e0": e0, "e1": e1, "norm": norm, "setup_s": setup_s, "nsteps_total": nsteps_total, "tuned": tuned}
 
 
 def accuracy_run(args, prec, psi):
@@ -270,6 +271,9 @@ def main():
     ap.add_argument("--ensemble-runs", type=int, default=32)
     ap.add_argument("--ensemble-steps", type=int, default=60,
                     help="timed steps of the batched-ensemble leg (0: skip)")
+    ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
+                    help="before timing, let hetd_autotune pick among the bit-identical kernel variants, "
+                         "timing each over STEPS steps (0: off, the default kernels)")
     ap.add_argument("--check-steps", type=int, default=400,
                     help="untimed mixed-vs-FP64 accuracy run length (0: skip)")
     args = ap.parse_args()
@@ -313,7 +317,9 @@ def main():
                                                    (mstep * 1e-3) / HBM_PEAK, 4),
                        "model_bytes_per_point_step": round(model_bytes_per_point_step(n, k == "mixed"), 2),
                        "E_per_atom_end": v["e1"]["etot4"] / 1000.0, "norm_end": v["norm"],
-                       "kernel_classes": v["classes"]}
+                       "kernel_classes": v["classes"],
+                       "variants": None if v["tuned"] is None else
+                       {"split_exchange": bool(v["tuned"] & 1), "two_plane_update": bool(v["tuned"] & 2)}}
     cmp_ = {}
     if len(res) == 2:
         a64, amx = res["fp64"], res["mixed"]

@@ -153,6 +153,15 @@ int hetd_steppc(hetd_ctx *ctx, double deltat, double *errHe, double *errimp);
 int hetd_energy(hetd_ctx *ctx, hetd_energies *e);
 /* sum(den)*dxyz (DFT4He3d.f90:869). */
 int hetd_norm(hetd_ctx *ctx, double *norm);
+/* Autotuner (no counterpart in the reference; for callers that run many
+ * steps on one grid).  Times, over `steps` steps from the current state on
+ * this context's own grid, the kernel variants that compute the same bits:
+ * x/y register FFT passes with a complex or a split LDS exchange, and update
+ * kernels with one or two LDS planes.  Keeps the fastest for this context and
+ * leaves the state exactly as it was: every device buffer is copied aside
+ * during the call, so the context's device memory is needed twice.  *choice
+ * (may be NULL): bit 0 = split exchanges, bit 1 = two-plane update kernels. */
+int hetd_autotune(hetd_ctx *ctx, double deltat, int32_t steps, int32_t *choice);
 /* The diagnostics of the driver's energy block (DFT4He3d.f90:868-947), as
  * device reductions: out[0..2] = centre of mass sum(den*x)*dxyz/npart
  * (r_cm.f90:23-55), out[3..5] = sum(den*x^2)*dxyz (not about the centre),

@@ -142,3 +142,23 @@ def test_pingpong_update_variant_is_bitwise(shape, prec, icon):
         same_state(a, b)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("shape,prec,icon", [((32, 32, 64), capi.HETD_FP64, 13), ((30, 32, 16), capi.HETD_MIXED, 8)])
+def test_autotune_leaves_state_and_results_unchanged(shape, prec, icon):
+    """hetd_autotune times variants that compute the same bits and restores
+    the state: afterwards the context evolves bit for bit as an untuned one."""
+    say("autotune", shape, prec, icon)
+    a = make(shape, prec, icon, graphs=True)
+    b = make(shape, prec, icon, graphs=True)
+    for c in (a, b):
+        c.evolve(G.DT, 5, traj=False)
+    choice = b.autotune(G.DT, 4)
+    assert 0 <= choice <= 3
+    same_state(a, b)
+    for n in (6, 9):
+        a.evolve(G.DT, n, traj=False)
+        b.evolve(G.DT, n, traj=False)
+        same_state(a, b)
+    a.close()
+    b.close()
