@@ -1302,7 +1302,8 @@ int hetd_pool_evolve(hetd_ctx* const* ctxs, int32_t n, double dt, int32_t nsteps
 // Chooses, by timing on the context's own grid and state, among kernel
 // variants that compute the same bits, so that tuning never changes a result:
 // the x/y register passes with a complex or a split LDS exchange, and the
-// update kernels with one or two LDS planes.  Every device buffer is copied
+// update kernels with one or two LDS planes, the latter also with
+// non-temporal loads and stores of the operands used once.  Every device buffer is copied
 // aside first and copied back after each trial, with the host-side step state.
 
 int hetd_autotune(hetd_ctx* c, double dt, int32_t steps, int32_t* choice) {
@@ -1357,9 +1358,10 @@ int hetd_autotune(hetd_ctx* c, double dt, int32_t steps, int32_t* choice) {
         if (hipEventCreate(&e) != hipSuccess) return give_up(fail(c, HETD_ERR_HIP, "hetd_autotune: events"));
     float best = 0.0f;
     int bestv = 0;
-    for (int v = 0; v < 4; ++v) {
+    static const int kTile[3] = {0, 4, 5};  // default, two LDS planes, and non-temporal operand streams
+    for (int v = 0; v < 6; ++v) {
         c->xsplit = (v & 1) != 0;
-        c->upd_tile = (v & 2) ? 4 : 0;
+        c->upd_tile = kTile[v >> 1];
         drop_graph(c);
         // an untimed pass first: it builds this variant's step graph
         if (int rc = hetd_evolve(c, dt, steps, nullptr)) return give_up(rc);
@@ -1377,7 +1379,7 @@ int hetd_autotune(hetd_ctx* c, double dt, int32_t steps, int32_t* choice) {
         }
     }
     c->xsplit = (bestv & 1) != 0;
-    c->upd_tile = (bestv & 2) ? 4 : 0;
+    c->upd_tile = kTile[bestv >> 1];
     drop_graph(c);
     cleanup();
     if (choice) *choice = bestv;

@@ -38,6 +38,30 @@ template <typename T> __device__ __forceinline__ T solid_penalty(T d, T csol, T 
 // reference's abs(z)**2); the two differ by at most an ulp or two
 __device__ __forceinline__ double abs2(double2 v) { return fma(v.x, v.x, v.y * v.y); }
 __device__ __forceinline__ double cabs_(double2 v) { return sqrt(abs2(v)); }
+// Streams read or written once per kernel: non-temporal (streaming) loads
+// and stores, which let the caches keep the data that is read again.
+typedef double hetd_d2v __attribute__((ext_vector_type(2)));
+typedef float hetd_f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_nt(const double2* p) {
+    const hetd_d2v t = __builtin_nontemporal_load(reinterpret_cast<const hetd_d2v*>(p));
+    return make_double2(t[0], t[1]);
+}
+__device__ __forceinline__ float2 ld_nt(const float2* p) {
+    const hetd_f2v t = __builtin_nontemporal_load(reinterpret_cast<const hetd_f2v*>(p));
+    return make_float2(t[0], t[1]);
+}
+__device__ __forceinline__ void st_nt(double2* p, double2 v) {
+    hetd_d2v t;
+    t[0] = v.x;
+    t[1] = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<hetd_d2v*>(p));
+}
+__device__ __forceinline__ void st_nt(float2* p, float2 v) {
+    hetd_f2v t;
+    t[0] = v.x;
+    t[1] = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<hetd_f2v*>(p));
+}
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmk<C>(a.x + b.x, a.y + b.y); }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmk<C>(a.x - b.x, a.y - b.y); }
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) { return cmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }

@@ -82,9 +82,24 @@ __device__ __forceinline__ double dens(double2 v) { return abs2(v); }
 // RHS history hpsiold: complex128, or complex64 in mixed precision
 __device__ __forceinline__ double2 widen2(float2 v) { return make_double2((double)v.x, (double)v.y); }
 __device__ __forceinline__ double2 widen2(double2 v) { return v; }
-template <bool MIX> __device__ __forceinline__ void put_hist(double2* h, long o, double2 v) {
-    if constexpr (MIX) reinterpret_cast<float2*>(h)[o] = make_float2((float)v.x, (float)v.y);
-    else h[o] = v;
+template <bool MIX, bool NT = false> __device__ __forceinline__ void put_hist(double2* h, long o, double2 v) {
+    if constexpr (MIX) {
+        const float2 f = make_float2((float)v.x, (float)v.y);
+        if constexpr (NT) st_nt(reinterpret_cast<float2*>(h) + o, f);
+        else reinterpret_cast<float2*>(h)[o] = f;
+    } else {
+        if constexpr (NT) st_nt(h + o, v);
+        else h[o] = v;
+    }
+}
+// a store of a once-used stream (NT: non-temporal)
+template <bool NT> __device__ __forceinline__ void put2(double2* p, long o, double2 v) {
+    if constexpr (NT) st_nt(p + o, v);
+    else p[o] = v;
+}
+template <bool NT, typename T> __device__ __forceinline__ T get2(const T* p, long o) {
+    if constexpr (NT) return ld_nt(p + o);
+    else return p[o];
 }
 template <bool MIX> __device__ __forceinline__ void put_den(const UpdateArgs& a, long o, double d) {
     if constexpr (MIX) a.denf[o] = (float)d;
@@ -98,8 +113,12 @@ template <bool MIX> __device__ __forceinline__ void put_den(const UpdateArgs& a,
 // forceimp (potenimp.f90:24-55) of the state the update starts from.
 // MIX (UPDATE only): FP32 mean field in, FP32 den and complex64 RHS history.
 // SK (UPDATE only): the stage, STAGE_RK / STAGE_PRED / STAGE_CORR.
-template <int KIND, int M, int TPY, int TNT, bool PF, bool FRC, bool MIX, int SK, bool PP>
+// OPT (UPDATE variants): OPT_PP ping-pong LDS planes, OPT_NT non-temporal
+// loads and stores of the once-used update operands.
+enum { OPT_PP = 1, OPT_NT = 2 };
+template <int KIND, int M, int TPY, int TNT, bool PF, bool FRC, bool MIX, int SK, int OPT>
 __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KIND>::A args) {
+    constexpr bool PP = (OPT & OPT_PP) != 0, NT = (OPT & OPT_NT) != 0;
     using V = typename TArgs<KIND>::V;
     constexpr int NC = TArgs<KIND>::NC;
     constexpr int RS = TNT / TTX;        // thread rows
@@ -248,13 +267,13 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
             if constexpr (MIX) opP[p] = a.pot4f[o];
             else opP[p] = a.pot4[o];
             if constexpr (SK == STAGE_RK) {
-                opA[p] = a.q[o];
+                opA[p] = get2<NT>(a.q, o);
             } else {
-                opA[p] = a.po3[o];
-                opD[p] = a.pc[o];
-                opH1[p] = reinterpret_cast<const HT*>(a.h1)[o];
-                opH2[p] = reinterpret_cast<const HT*>(a.h2)[o];
-                if constexpr (SK == STAGE_CORR) opB[p] = a.po2[o];
+                opA[p] = get2<NT>(a.po3, o);
+                opD[p] = get2<NT>(a.pc, o);
+                opH1[p] = get2<NT>(reinterpret_cast<const HT*>(a.h1), o);
+                opH2[p] = get2<NT>(reinterpret_cast<const HT*>(a.h2), o);
+                if constexpr (SK == STAGE_CORR) opB[p] = get2<NT>(a.po2, o);
             }
         }
     };
@@ -409,12 +428,12 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     s1.y = a.ra * (fv.y - a.rb * qq.y);
                     qn.x = (qq.x + 3.0 * s1.x) - a.rc * fv.x;
                     qn.y = (qq.y + 3.0 * s1.y) - a.rc * fv.y;
-                    a.q[o] = qn;
-                    if (a.rk_j == 0) put_hist<MIX>(a.hnew, o, fv);
+                    put2<NT>(a.q, o, qn);
+                    if (a.rk_j == 0) put_hist<MIX, NT>(a.hnew, o, fv);
                     pn.x = ps.x + a.dt * s1.x;
                     pn.y = ps.y + a.dt * s1.y;
                     const double d = abs2(pn);
-                    a.psi_out[o] = pn;
+                    put2<NT>(a.psi_out, o, pn);
                     put_den<MIX>(a, o, d);
                     if (!(d > 0.0)) atomicOr(a.guard, 1);
                 } else if constexpr (SK == STAGE_PRED) {
@@ -427,9 +446,9 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     const double c112 = 112.0 / 121.0;
                     pn.x = P.x - c112 * pc.x;
                     pn.y = P.y - c112 * pc.y;
-                    a.psi_out[o] = pn;
-                    a.pc[o] = P;
-                    put_hist<MIX>(a.hnew, o, fv);
+                    put2<NT>(a.psi_out, o, pn);
+                    put2<NT>(a.pc, o, P);
+                    put_hist<MIX, NT>(a.hnew, o, fv);
                     const double d = abs2(pn);
                     put_den<MIX>(a, o, d);
                 } else {
@@ -444,8 +463,8 @@ __global__ void __launch_bounds__(TNT) k_tiled(TiledCommon tc, typename TArgs<KI
                     const double c9 = 9.0 / 121.0;
                     pn.x = Cc.x + c9 * pcn.x;
                     pn.y = Cc.y + c9 * pcn.y;
-                    a.pc[o] = pcn;
-                    a.psi_out[o] = pn;
+                    put2<NT>(a.pc, o, pcn);
+                    put2<NT>(a.psi_out, o, pn);
                     const double d = abs2(pn);
                     put_den<MIX>(a, o, d);
                     acc[0] += cabs_(pcn);
@@ -537,19 +556,21 @@ bool tiled_supported(const StencilCoef& sc) { return sc.m == 4 || sc.m == 6; }
 // issued before the current plane is computed beat 512-thread 64x8 tiles by
 // 0.15 ms/step.  HETD_TILE_FORCE / HETD_TILE_ENERGY = 0..4 select another
 // variant for those kinds; the update kernels (one per stage) are built for
-// variant 3 and, selected by HETD_TILE_UPDATE=4, variant 4 (variant 3 with
-// ping-pong planes: one barrier per plane instead of two).
-struct TileVariant { int tpy, tnt; bool pf, pp; };
-static constexpr int kNVariants = 5;
-static constexpr TileVariant kVariants[kNVariants] = {
-    {1, 512, false, false}, {1, 256, false, false}, {1, 512, true, false}, {1, 256, true, false}, {1, 256, true, true}};
+// variant 3 and, selected by HETD_TILE_UPDATE=4 or 5, variant 4 (variant 3
+// with ping-pong planes: one barrier per plane instead of two) and variant 5
+// (4 with non-temporal loads and stores of the once-used operands).
+struct TileVariant { int tpy, tnt; bool pf; int opt; };
+static constexpr int kNVariants = 6;
+static constexpr TileVariant kVariants[kNVariants] = {{1, 512, false, 0}, {1, 256, false, 0},
+                                                      {1, 512, true, 0},  {1, 256, true, 0},
+                                                      {1, 256, true, OPT_PP}, {1, 256, true, OPT_PP | OPT_NT}};
 
 static constexpr int kUpdateVariant = 3;  // the default update kernels
 
 static int variant_for(int kind) {
     static const char* names[3] = {"HETD_TILE_UPDATE", "HETD_TILE_FORCE", "HETD_TILE_ENERGY"};
     const char* e = getenv(names[kind]);
-    if (kind == TK_UPDATE) return e && e[0] == '4' ? 4 : kUpdateVariant;
+    if (kind == TK_UPDATE) return e && (e[0] == '4' || e[0] == '5') ? e[0] - '0' : kUpdateVariant;
     if (e && e[0] >= '0' && e[0] < '0' + kNVariants) return e[0] - '0';
     return 0;
 }
@@ -557,8 +578,10 @@ static int variant_for(int kind) {
 template <int KIND, int M, int V, bool FRC, bool MIX, int SK>
 static void launch_k(const TiledCommon& tc, const typename TArgs<KIND>::A& a, int nlaunch, hipStream_t st) {
     constexpr TileVariant tv = kVariants[V];
-    hipLaunchKernelGGL((k_tiled<KIND, M, tv.tpy, tv.tnt, tv.pf, FRC, MIX, SK, tv.pp>), dim3(nlaunch), dim3(tv.tnt),
-                       0, st, tc, a);
+    // the NT option concerns the update operands only
+    constexpr int opt = KIND == TK_UPDATE ? tv.opt : (tv.opt & OPT_PP);
+    hipLaunchKernelGGL((k_tiled<KIND, M, tv.tpy, tv.tnt, tv.pf, FRC, MIX, SK, opt>), dim3(nlaunch), dim3(tv.tnt), 0,
+                       st, tc, a);
 }
 
 template <int M, int V, bool FRC, bool MIX>
@@ -590,6 +613,7 @@ template <int KIND, int M>
 static void launch_m(int v, const TiledCommon& tc, const typename TArgs<KIND>::A& a, int nlaunch, hipStream_t st) {
     if constexpr (KIND == TK_UPDATE) {
         if (v == 4) launch_v<KIND, M, 4>(tc, a, nlaunch, st);
+        else if (v == 5) launch_v<KIND, M, 5>(tc, a, nlaunch, st);
         else launch_v<KIND, M, kUpdateVariant>(tc, a, nlaunch, st);
     } else {
         switch (v) {
@@ -606,7 +630,7 @@ template <int KIND>
 static hipError_t launch_tiled(const typename TArgs<KIND>::A& a, hipStream_t st, int* nblocks) {
     int v = variant_for(KIND);
     if constexpr (KIND == TK_UPDATE)
-        if (a.tile == 4 || a.tile == kUpdateVariant) v = a.tile;
+        if (a.tile == 4 || a.tile == 5 || a.tile == kUpdateVariant) v = a.tile;
     const TileVariant tv = kVariants[v];
     TiledCommon tc = tiled_common(a.g, a.sc, tv.tpy * tv.tnt / TTX);
     const int nb = tc.ntx * tc.nty * tc.ntz;

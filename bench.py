@@ -319,7 +319,8 @@ def main():
                        "E_per_atom_end": v["e1"]["etot4"] / 1000.0, "norm_end": v["norm"],
                        "kernel_classes": v["classes"],
                        "variants": None if v["tuned"] is None else
-                       {"split_exchange": bool(v["tuned"] & 1), "two_plane_update": bool(v["tuned"] & 2)}}
+                       {"split_exchange": bool(v["tuned"] & 1),
+                        "update_kernels": ["default", "two LDS planes", "two LDS planes, non-temporal"][v["tuned"] >> 1]}}
     cmp_ = {}
     if len(res) == 2:
         a64, amx = res["fp64"], res["mixed"]

@@ -157,10 +157,12 @@ int hetd_norm(hetd_ctx *ctx, double *norm);
  * steps on one grid).  Times, over `steps` steps from the current state on
  * this context's own grid, the kernel variants that compute the same bits:
  * x/y register FFT passes with a complex or a split LDS exchange, and update
- * kernels with one or two LDS planes.  Keeps the fastest for this context and
- * leaves the state exactly as it was: every device buffer is copied aside
- * during the call, so the context's device memory is needed twice.  *choice
- * (may be NULL): bit 0 = split exchanges, bit 1 = two-plane update kernels. */
+ * kernels with one or two LDS planes (and, with two, with non-temporal
+ * operand streams).  Keeps the fastest for this context and leaves the state
+ * exactly as it was: every device buffer is copied aside during the call, so
+ * the context's device memory is needed twice.  *choice (may be NULL): bit 0
+ * = split exchanges; bits 1-2 = update kernels (0 default, 1 two planes,
+ * 2 two planes and non-temporal streams). */
 int hetd_autotune(hetd_ctx *ctx, double deltat, int32_t steps, int32_t *choice);
 /* The diagnostics of the driver's energy block (DFT4He3d.f90:868-947), as
  * device reductions: out[0..2] = centre of mass sum(den*x)*dxyz/npart
