@@ -324,8 +324,8 @@ static int term_alfa_dev(hetd_ctx* c) {
     if ((rc = deriv_dev(c, 1, 1, 0, c->s1, c->s4))) return rc;
     if ((rc = deriv_dev(c, 1, 2, 0, c->s2, c->s5))) return rc;
     if ((rc = deriv_dev(c, 1, 3, 0, c->s3, c->s6))) return rc;
-    HCHK(c, launch_ualpha_final(c->ualphas, c->s4, c->s5, c->s6, c->p.den0s, c->p.h2o2m4, c->p.alphas, c->pot4, n,
-                                c->st));
+    HCHK(c, launch_ualpha_final(c->ualphas, c->s4, c->s5, c->s6, c->p.den0s, c->p.h2o2m4, c->p.alphas, c->pot4,
+                                c->mixed ? c->pot4_32 : nullptr, n, c->st));
     return 0;
 }
 
@@ -496,11 +496,25 @@ static int ensure_delj4(hetd_ctx* c) {
     return 0;
 }
 
+static int ensure_den64(hetd_ctx* c);
+
 static int poten_dev(hetd_ctx* c, bool want_ccor = false, bool want_fden = false, bool keep_pot4 = true,
                      bool full_form = false) {
+    if (c->mixed) {
+        // the alpha_s term (OTC) stays FP64: coarse-grained density
+        // IFFT(K_alpha FFT(den)) and term_alfa from the FP64 den of psi,
+        // added to the FP32 mean field
+        int rc = poten_t<float>(c, pf32(c), want_ccor || want_fden || full_form, want_ccor, want_fden, keep_pot4);
+        if (rc || !c->otc || !keep_pot4) return rc;
+        if ((rc = ensure_den64(c))) return rc;
+        const double* in[1] = {c->den};
+        double2* wk[1] = {c->Cs};
+        double* out[1] = {c->denalf};
+        if ((rc = convolve_dev(c, 1, in, wk, out, c->kalfs))) return rc;
+        return term_alfa_dev(c);
+    }
     // OTC keeps the full form: its K_alpha spectrum rides on the first z pass
     const bool full = want_ccor || want_fden || c->otc || full_form;
-    if (c->mixed) return poten_t<float>(c, pf32(c), full, want_ccor, want_fden, keep_pot4);
     int rc = poten_t<double>(c, pf64(c), full, want_ccor, want_fden, keep_pot4);
     if (rc) return rc;
     if (c->otc && keep_pot4) {
@@ -832,8 +846,6 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
     else if (p->npd < 3 || p->npd > 25 || p->npd % 2 == 0) rc = bad("npd must be odd, 3..25 (newder.f:75-76)");
     else if (p->icon != 13 && p->icon != 8 && p->icon != 18) rc = bad("icon must be 13, 8 or 18");
     else if (p->precision != HETD_FP64 && p->precision != HETD_MIXED) rc = bad("precision must be HETD_FP64 or HETD_MIXED");
-    else if (p->precision == HETD_MIXED && !p->lsolid && p->core == HETD_CORE_OTC)
-        rc = bad("HETD_MIXED supports the OT core (the alpha_s term runs in FP64 only)");
     else if (p->precision == HETD_MIXED && p->npd != 9 && p->npd != 13) rc = bad("HETD_MIXED needs npd 9 or 13");
     else if (p->nx > 1024 || p->ny > 1024 || p->nz > 1024) rc = bad("grid edge above 1024");
     if (rc) { *out = c; return rc; }

@@ -272,7 +272,7 @@ hipError_t launch_mul(const double* a, const double* b, double* out, long n, hip
 hipError_t launch_dot3(const double* a1, const double* b1, const double* a2, const double* b2,
                        const double* a3, const double* b3, double* out, long n, hipStream_t st);
 hipError_t launch_ualpha_final(double* ua, const double* s4, const double* s5, const double* s6,
-                               double den0s, double h2o2m4, double alphas, double* pot4, long n, hipStream_t st);
+                               double den0s, double h2o2m4, double alphas, double* pot4, float* pot4f, long n, hipStream_t st);
 hipError_t launch_scale(double* a, double s, long n, hipStream_t st);
 
 }  // namespace hetd

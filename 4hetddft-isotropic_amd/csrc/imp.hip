@@ -298,18 +298,21 @@ hipError_t launch_dot3(const double* a1, const double* b1, const double* a2, con
 
 // ualphas = (ualphas/den0s + s4 + s5 + s6)*h2o2m4*alphas; pot4 += ualphas
 // (term_alfa.f90:98-113, poten.f90:103-106)
+// pot4 (FP64) or pot4f (the FP32 mean field of the mixed path) += u_alpha
 __global__ void k_ualpha_final(double* ua, const double* s4, const double* s5, const double* s6, double den0s,
-                               double h2, double al, double* pot4, long n) {
+                               double h2, double al, double* pot4, float* pot4f, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const double u = (((((ua[i] / den0s) + s4[i]) + s5[i]) + s6[i]) * h2) * al;
         ua[i] = u;
-        pot4[i] = pot4[i] + u;
+        if (pot4f) pot4f[i] = (float)((double)pot4f[i] + u);
+        else pot4[i] = pot4[i] + u;
     }
 }
 hipError_t launch_ualpha_final(double* ua, const double* s4, const double* s5, const double* s6, double den0s,
-                               double h2o2m4, double alphas, double* pot4, long n, hipStream_t st) {
-    hipLaunchKernelGGL(k_ualpha_final, grid1(n), dim3(256), 0, st, ua, s4, s5, s6, den0s, h2o2m4, alphas, pot4, n);
+                               double h2o2m4, double alphas, double* pot4, float* pot4f, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_ualpha_final, grid1(n), dim3(256), 0, st, ua, s4, s5, s6, den0s, h2o2m4, alphas, pot4,
+                       pot4f, n);
     return hipGetLastError();
 }
 

@@ -184,6 +184,35 @@ def test_evolve_matches_oracle_other_grids(oracle_mod, shape, npd, icon, core):
         assert abs(e1[k] - e2[k]) <= TOL * max(abs(e2[k]), 1.0), k
 
 
+@pytest.mark.parametrize("shape,icon", [((36, 32, 40), 8), ((40, 36, 32), 13)])
+def test_mixed_precision_otc_tracks_fp64(shape, icon):
+    """HETD_MIXED with the OTC core: the FP32 mean field plus the alpha_s
+    term formed in FP64 from the FP64 density, against the FP64 path
+    (relative energy error <= 1e-6, psi within 1e-5)."""
+    say("mixed OTC", shape, icon)
+    from hetd.initstate import woods_saxon_state
+    nz, ny, nx = shape
+    h = 0.6
+    kw = dict(nx=nx, ny=ny, nz=nz, xmax=nx * h / 2, ymax=ny * h / 2, zmax=nz * h / 2, npd=13, icon=icon,
+              core="OTC", lsolid=0)
+    psi = woods_saxon_state(nx, ny, nz, kw["xmax"], kw["ymax"], kw["zmax"], 150, 3)
+    out = {}
+    for prec in (capi.HETD_FP64, capi.HETD_MIXED):
+        p = capi.default_params(precision=prec, **kw)
+        with Context(p, selec="Xe_He", r_cutoff=2.55, umax=2793.9365463) as ctx:
+            ctx.set_state(psi, (0.3, -0.2, 0.1), (1.0, 2.0, -3.0))
+            ctx.init()
+            t = ctx.evolve(G.DT, 12)
+            out[prec] = (t, ctx.field(capi.F_PSI), ctx.energy(), ctx.norm())
+    t64, p64, e64, n64 = out[capi.HETD_FP64]
+    t32, p32, e32, n32 = out[capi.HETD_MIXED]
+    assert G.rel(p32, p64) < 1e-5
+    for k in ("etot4", "etot", "ealphas"):
+        assert abs(e32[k] - e64[k]) <= 1e-6 * abs(e64[k]), k
+    assert abs(n32 - n64) <= 1e-9 * n64
+    assert np.abs(t32[:, 2:5] - t64[:, 2:5]).max() < 1e-8 * max(np.abs(t64[:, 2:5]).max(), 1.0)
+
+
 @pytest.mark.parametrize("selec,r_cutoff,umax", [("RB", 3.0, 1.0e3), ("Ba_plus_gs", 1.05, 6.836177e3),
                                                  ("Ag_gs", 2.0, 5.0e3)])
 def test_evolve_other_impurity_potentials(oracle_mod, selec, r_cutoff, umax):
