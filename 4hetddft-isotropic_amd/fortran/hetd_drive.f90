@@ -49,6 +49,9 @@ program hetd_drive
   ! extension (not a reference key): keep the integrator history beside every
   ! checkpoint (<file>.hist) and restart from <filedenin>.hist when present
   logical :: lhistory = .false.
+  ! extension (not a reference key): run the mixed-precision path (FP32 mean
+  ! field and FFTs, FP64 wave function and integrator state)
+  logical :: lmixed = .false.
 
   namelist /input/title,fftwplan,nthread,nsfiles,                         &
                   fileout,filemonopol, filerimp,filevimp,fileaimp,        &
@@ -69,7 +72,7 @@ program hetd_drive
                   pcurr,icurr,lsurf,lsurf3D,Lambdah,                      &
                   txmean,txsurf,tymean,tysurf,tzmean,tzsurf,              &
                   vximp,vyimp,vzimp,selec,r_cutoff,umax,                  &
-                  selec_gs,r_cutoff_gs,umax_gs,lhistory
+                  selec_gs,r_cutoff_gs,umax_gs,lhistory,lmixed
 
   real(8), parameter :: ktops = 7.63822291d0, pstok = 1.d0/7.63822291d0, mp_u = 0.02061484d0
   type(hetd_params) :: p
@@ -112,7 +115,7 @@ program hetd_drive
   call hetd_default_params(p)
   p%nx = nx; p%ny = ny; p%nz = nz; p%npd = npd; p%icon = icon
   p%core = merge(HETD_CORE_OTC, HETD_CORE_OT, core4 == 'OTC')
-  p%lsolid = merge(1, 0, lsolid); p%precision = HETD_FP64
+  p%lsolid = merge(1, 0, lsolid); p%precision = merge(HETD_MIXED, HETD_FP64, lmixed)
   p%xmax = xmax; p%ymax = ymax; p%zmax = zmax
   p%eps4 = eps4; p%sigma4 = sigma4; p%cp4 = cp4; p%cpp4 = cpp4; p%h2o2m4 = h2o2m4
   p%alphas = alphas; p%l_alpha = l

@@ -186,3 +186,22 @@ def test_driver_mode0_wavefunction_and_mode3(tmp_path):
     # carries 11 significant digits
     assert np.abs(out["m0"][0] - out["m2"][0]).max() < 2e-10 * np.abs(out["m2"][0]).max()
     assert np.abs(out["m0"][1] - out["m2"][1]).max() < 1e-12 * np.abs(out["m2"][1]).max()
+
+
+@pytest.mark.gpu
+def test_driver_mixed_precision_key(tmp_path):
+    """lmixed (an extension key) runs the reference driver on the mixed-precision
+    path; its impurity trajectory and final state track the FP64 run's."""
+    cfg, d = G.load("ot_xe_32")
+    runs = {}
+    for key in (".false.", ".true."):
+        w = os.path.join(tmp_path, "m" + key.strip("."))
+        os.makedirs(w)
+        write_mode2(os.path.join(w, "den.in"), d["psi0"], cfg["xmax"], cfg["ymax"], cfg["zmax"],
+                    d["rimp0"], d["vimp0"])
+        r = run_drive(w, namelist(cfg, lmixed=key))
+        assert r.returncode == 0, r.stderr
+        runs[key] = (np.loadtxt(os.path.join(w, "rimp.out"), comments="#"), read_mode2(os.path.join(w, "den.out"))[0])
+    (r64, p64), (r32, p32) = runs[".false."], runs[".true."]
+    assert np.abs(r32 - r64).max() < 1e-8 * np.abs(r64).max()
+    assert G.rel(p32, p64) < 1e-5
