@@ -39,7 +39,7 @@ extern "C" {
 #define HETD_CORE_OT 0
 #define HETD_CORE_OTC 1
 
-/* field identifiers for hetd_get_field / hetd_put_field */
+/* field identifiers for hetd_get_field */
 #define HETD_F_PSI 0      /* complex(8) psi */
 #define HETD_F_DEN 1      /* den = |psi|^2 */
 #define HETD_F_POT4 2     /* mean field U (poten.f90:90-114) */
@@ -65,10 +65,10 @@ extern "C" {
 typedef struct hetd_params {
     int32_t nx, ny, nz;   /* grid points (namelist nx, ny, nz) */
     int32_t npd;          /* stencil points, odd 3..25 (modules.f90:30) */
-    int32_t icon;         /* 13: zero outside the box; 8: periodic (modules.f90:34) */
+    int32_t icon;         /* 13: zero outside the box; 8: periodic; 18: antiperiodic (modules.f90:34) */
     int32_t core;         /* HETD_CORE_OT / HETD_CORE_OTC */
     int32_t lsolid;       /* solid penalty; forces core OT (DFT4He3d.f90:476) */
-    int32_t precision;    /* HETD_FP64 or HETD_MIXED */
+    int32_t precision;    /* HETD_FP64 or HETD_MIXED (npd 9 or 13) */
     double xmax, ymax, zmax;      /* box half-widths, h = 2|xmax|/nx (DFT4He3d.f90:274) */
     /* Orsay-Trento functional (modules.f90:130-150, 204-216) */
     double eps4, sigma4, h4;      /* LJ well, size, core radius (h4ot) */
