@@ -72,6 +72,7 @@ struct hetd_ctx {
     double* dtmp[2] = {nullptr, nullptr};  // scratch of hetd_deriv / hetd_fft_* (allocated on first use)
     int ioldp[3] = {1, 2, 3}, ioldh[2] = {1, 2}, ioldr[3] = {1, 2, 3}, ioldv[3] = {1, 2, 3}, iolda[2] = {1, 2};
     long iter_since_set = 0;
+    bool ready = false;  // hetd_init completed
     bool have_state = false;
     // predictor-corrector steps replayed from a HIP graph (see step_graph)
     bool use_graphs = true;
@@ -528,7 +529,6 @@ static int poten_dev(hetd_ctx* c, bool want_ccor = false, bool want_fden = false
     return 0;
 }
 
-// forceimp + aimp = F/mAg_u on the device (potenimp.f90:24-55, DFT4He3d.f90:852)
 // FP64 den from psi where the mixed path has written only the FP32 copy
 static int ensure_den64(hetd_ctx* c) {
     if (c->den64_ok) return 0;
@@ -537,6 +537,7 @@ static int ensure_den64(hetd_ctx* c) {
     return 0;
 }
 
+// forceimp + aimp = F/mAg_u on the device (potenimp.f90:24-55, DFT4He3d.f90:852)
 static int force_dev(hetd_ctx* c) {
     c->force_pending = false;
     if (int rc = ensure_den64(c)) return rc;
@@ -1018,6 +1019,7 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
         }
     }
     fill_grid(c);
+    c->ready = true;
     return HETD_OK;
 }
 
@@ -1037,6 +1039,7 @@ int hetd_finalize(hetd_ctx* c) {
 
 int hetd_set_potion(hetd_ctx* c, const double* potion, int32_t npot, double delinter) {
     if (!c || !potion || npot < 2 || !(delinter > 0)) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     if (c->potion && c->npot != npot) return fail(c, HETD_ERR_ARG, "table size changed");
     if (!c->potion) HCHK(c, dalloc(c, &c->potion, sizeof(double) * npot));
     HCHK(c, hipMemcpyAsync(c->potion, potion, sizeof(double) * npot, hipMemcpyHostToDevice, c->st));
@@ -1056,6 +1059,7 @@ int hetd_vsurf(int32_t lsurf3d, double x, double y, double z, double* v) {
 
 int hetd_set_uext(hetd_ctx* c, const double* uext) {
     if (!c) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     if (!uext) {
         c->uext_on = false;
         return HETD_OK;
@@ -1069,6 +1073,7 @@ int hetd_set_uext(hetd_ctx* c, const double* uext) {
 
 int hetd_set_surface(hetd_ctx* c, int32_t lsurf3d, double zsurf) {
     if (!c) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     const int nx = c->nx, ny = c->ny, nz = c->nz;
     const hetd_params& p = c->p;
     std::vector<double> u((size_t)c->n);
@@ -1088,6 +1093,7 @@ int hetd_set_surface(hetd_ctx* c, int32_t lsurf3d, double zsurf) {
 
 int hetd_set_state(hetd_ctx* c, const double* psi, const double r[3], const double v[3]) {
     if (!c || !psi || !r || !v) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     if (!c->potion) return fail(c, HETD_ERR_STATE, "hetd_set_potion must precede hetd_set_state");
     drop_graph(c);
     const size_t nC = sizeof(double2) * c->n;
@@ -1138,10 +1144,11 @@ static size_t history_bytes(const hetd_ctx* c) {
     return sizeof(HistHeader) + sizeof(ImpState) + 5 * nC + 2 * nHist;
 }
 
-int64_t hetd_history_size(hetd_ctx* c) { return c ? (int64_t)history_bytes(c) : -1; }
+int64_t hetd_history_size(hetd_ctx* c) { return c && c->ready ? (int64_t)history_bytes(c) : -1; }
 
 int hetd_save_history(hetd_ctx* c, void* host) {
     if (!c || !host) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
     if (int rc = settle_force(c)) return rc;
     HistHeader h{};
@@ -1169,6 +1176,7 @@ int hetd_save_history(hetd_ctx* c, void* host) {
 
 int hetd_load_history(hetd_ctx* c, const void* host, int64_t bytes) {
     if (!c || !host) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     if (!c->have_state) return fail(c, HETD_ERR_STATE, "hetd_set_state must precede hetd_load_history");
     HistHeader h;
     if (bytes < (int64_t)sizeof h) return fail(c, HETD_ERR_ARG, "history blob too short");
@@ -1201,6 +1209,7 @@ int hetd_load_history(hetd_ctx* c, const void* host, int64_t bytes) {
 
 int hetd_potenimp(hetd_ctx* c, const double r[3]) {
     if (!c || !r) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     HCHK(c, hipMemcpyAsync(c->imp->rpot, r, 3 * sizeof(double), hipMemcpyHostToDevice, c->st));
     HCHK(c, hipStreamSynchronize(c->st));
     return HETD_OK;
@@ -1208,6 +1217,7 @@ int hetd_potenimp(hetd_ctx* c, const double r[3]) {
 
 int hetd_poten(hetd_ctx* c) {
     if (!c) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
     // called on its own: the full form, so that every field of poten is formed
     int rc = poten_dev(c, false, false, true, true);
@@ -1219,6 +1229,7 @@ int hetd_poten(hetd_ctx* c) {
 int hetd_forceimp(hetd_ctx* c, const double r[3], double F[3]) {
     (void)r;  // the reference's forceimp uses the u_imp of the last potenimp
     if (!c || !F) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
     int rc = force_dev(c);
     if (rc) return rc;
@@ -1229,6 +1240,7 @@ int hetd_forceimp(hetd_ctx* c, const double r[3], double F[3]) {
 
 int hetd_set_aimp(hetd_ctx* c, const double a[3]) {
     if (!c || !a) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     HCHK(c, hipMemcpyAsync(c->imp->a, a, 3 * sizeof(double), hipMemcpyHostToDevice, c->st));
     HCHK(c, hipStreamSynchronize(c->st));
     return HETD_OK;
@@ -1236,6 +1248,7 @@ int hetd_set_aimp(hetd_ctx* c, const double a[3]) {
 
 int hetd_steprk(hetd_ctx* c, double dt) {
     if (!c) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
     int rc = steprk_dev(c, dt, false);
     if (rc) return rc;
@@ -1245,6 +1258,7 @@ int hetd_steprk(hetd_ctx* c, double dt) {
 
 int hetd_steppc(hetd_ctx* c, double dt, double* errHe, double* errimp) {
     if (!c || !errHe || !errimp) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
     int rc = steppc_dev(c, dt, false);
     if (rc) return rc;
@@ -1259,6 +1273,7 @@ int hetd_steppc(hetd_ctx* c, double dt, double* errHe, double* errimp) {
 
 int hetd_evolve(hetd_ctx* c, double dt, int32_t nsteps, double* traj) {
     if (!c || nsteps < 0) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
     if (traj && c->traj_cap < (long)nsteps * HETD_TRAJ_STRIDE) {
         if (c->traj) {
@@ -1286,6 +1301,7 @@ int hetd_pool_evolve(hetd_ctx* const* ctxs, int32_t n, double dt, int32_t nsteps
     if (!ctxs || n < 0 || nsteps < 0) return HETD_ERR_ARG;
     for (int i = 0; i < n; ++i) {
         if (!ctxs[i]) return HETD_ERR_ARG;
+        if (!ctxs[i]->ready) return HETD_ERR_STATE;
         if (!ctxs[i]->have_state) return fail(ctxs[i], HETD_ERR_STATE, "no state");
     }
     // units of work are issued round-robin, so every member's stream always
@@ -1320,6 +1336,7 @@ int hetd_pool_evolve(hetd_ctx* const* ctxs, int32_t n, double dt, int32_t nsteps
 
 int hetd_autotune(hetd_ctx* c, double dt, int32_t steps, int32_t* choice) {
     if (!c || steps < 1 || !(dt > 0.0)) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     if (!c->have_state) return fail(c, HETD_ERR_STATE, "hetd_set_state must precede hetd_autotune");
     if (c->tclass) return fail(c, HETD_ERR_STATE, "hetd_autotune with kernel-class timing on");
     HCHK(c, hipStreamSynchronize(c->st));
@@ -1400,6 +1417,7 @@ int hetd_autotune(hetd_ctx* c, double dt, int32_t steps, int32_t* choice) {
 
 int hetd_energy(hetd_ctx* c, hetd_energies* e) {
     if (!c || !e) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
     if (int rc = ensure_delj4(c)) return rc;
     if (int rc = ensure_dencg(c)) return rc;
@@ -1445,6 +1463,7 @@ int hetd_energy(hetd_ctx* c, hetd_energies* e) {
 
 int hetd_moments(hetd_ctx* c, double npart, double out[10]) {
     if (!c || !out || !(npart != 0.0)) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     if (!c->have_state) return fail(c, HETD_ERR_STATE, "no state");
     EnergyArgs a{};
     a.g = c->grid; a.sc = c->sc; a.psi = c->psi; a.den = c->den; a.dencg = c->dencg; a.delj4 = c->delj4;
@@ -1474,6 +1493,7 @@ int hetd_moments(hetd_ctx* c, double npart, double out[10]) {
 
 int hetd_norm(hetd_ctx* c, double* norm) {
     if (!c || !norm) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     EnergyArgs a{};
     a.g = c->grid; a.sc = c->sc; a.psi = c->psi; a.den = c->den; a.dencg = c->dencg; a.delj4 = c->delj4;
     a.imp = c->imp; a.partial = c->partial;
@@ -1490,6 +1510,7 @@ int hetd_norm(hetd_ctx* c, double* norm) {
 
 int hetd_get_imp(hetd_ctx* c, double r[3], double v[3], double a[3]) {
     if (!c || !r || !v || !a) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     ImpState is;
     HCHK(c, hipMemcpyAsync(&is, c->imp, sizeof is, hipMemcpyDeviceToHost, c->st));
     HCHK(c, hipStreamSynchronize(c->st));
@@ -1505,6 +1526,7 @@ static const double* widen(hetd_ctx* c, const float* f, long n, double* dst) {
 
 int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
     if (!c || !host) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     const size_t nR = sizeof(double) * c->n, nC = sizeof(double2) * c->n;
     const void* src = nullptr;
     size_t bytes = nR;
@@ -1587,6 +1609,7 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
 
 int hetd_fft_r2c(hetd_ctx* c, const double* in, double* out) {
     if (!c || !in || !out) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     HCHK(c, hipMemcpyAsync(c->tmp, in, sizeof(double) * c->n, hipMemcpyHostToDevice, c->st));
     int rc = r2c_dev(c, c->tmp, c->A);
     if (rc) return rc;
@@ -1598,6 +1621,7 @@ int hetd_fft_r2c(hetd_ctx* c, const double* in, double* out) {
 
 int hetd_fft_c2r(hetd_ctx* c, const double* in, double* out) {
     if (!c || !in || !out) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     HCHK(c, hipMemcpy2DAsync(c->A, sizeof(double2) * c->ldh, in, sizeof(double2) * c->nxh, sizeof(double2) * c->nxh,
                              (size_t)c->ny * c->nz, hipMemcpyHostToDevice, c->st));
     int rc = c2r_dev(c, c->A, c->tmp, c->renor);
@@ -1609,6 +1633,7 @@ int hetd_fft_c2r(hetd_ctx* c, const double* in, double* out) {
 
 int hetd_deriv(hetd_ctx* c, int32_t id, int32_t iv, int32_t is_complex, const double* f, double* df) {
     if (!c || !f || !df || id < 1 || id > 2 || iv < 1 || iv > 3) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     const size_t bytes = (is_complex ? sizeof(double2) : sizeof(double)) * c->n;
     for (int i = 0; i < 2; ++i)
         if (!c->dtmp[i]) HCHK(c, hipMalloc(&c->dtmp[i], sizeof(double2) * c->n));
@@ -1622,6 +1647,7 @@ int hetd_deriv(hetd_ctx* c, int32_t id, int32_t iv, int32_t is_complex, const do
 
 int hetd_timing(hetd_ctx* c, int32_t kclass) {
     if (!c || kclass < HETD_K_OFF || kclass > HETD_K_FORCE) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     HCHK(c, hipStreamSynchronize(c->st));
     c->tclass = kclass;
     c->tused = 0;
@@ -1632,6 +1658,7 @@ int hetd_timing(hetd_ctx* c, int32_t kclass) {
 
 int hetd_timing_read(hetd_ctx* c, double* total_ms, int64_t* launches, double* bytes) {
     if (!c || !total_ms || !launches || !bytes) return HETD_ERR_ARG;
+    if (!c->ready) return HETD_ERR_STATE;  // hetd_init failed; hetd_last_error says why
     HCHK(c, hipStreamSynchronize(c->st));
     double t = 0;
     for (size_t i = 0; i + 1 < c->tused; i += 2) {

@@ -93,7 +93,9 @@ void hetd_default_params(hetd_params *p);
 /* Allocate the device context: replaces dimen (dimen.f90:29-127), fftini
  * (fft.f90:5-76), Init_deriv_p (init_deriv_parallel.f90:51-83), fforma
  * (fforma.f90:11-62), initcg (initcg.f90:5-51) and kalfs
- * (DFT4He3d.f90:592-596). */
+ * (DFT4He3d.f90:592-596).  On failure *ctx still receives the context:
+ * hetd_last_error says why, hetd_finalize releases it, and every other call
+ * on it returns HETD_ERR_STATE. */
 int hetd_init(const hetd_params *p, hetd_ctx **ctx);
 /* Release everything (end of DFT4He3d.f90). */
 int hetd_finalize(hetd_ctx *ctx);

@@ -54,3 +54,32 @@ def test_default_params_match_python_mirror():
     q = capi.default_params()
     for name, _ in capi.Params._fields_:
         assert getattr(p, name) == getattr(q, name), name
+
+
+@pytest.mark.parametrize("field,value,needle", [
+    ("npd", 4, "npd must be odd"),
+    ("icon", 7, "icon must be"),
+    ("nx", 2048, "above 1024"),
+])
+def test_failed_init_leaves_an_inert_context(field, value, needle):
+    # the configuration checks come before any HIP call, so this runs
+    # without a GPU: the context reports why and refuses every other call
+    L = capi.load()
+    p = capi.default_params(nx=32, ny=32, nz=32, xmax=6.4, ymax=6.4, zmax=6.4)
+    setattr(p, field, value)
+    h = ctypes.c_void_p()
+    assert L.hetd_init(ctypes.byref(p), ctypes.byref(h)) == capi.HETD_ERR_ARG
+    assert h.value
+    assert needle in L.hetd_last_error(h).decode()
+    d = ctypes.c_double()
+    assert L.hetd_poten(h) == capi.HETD_ERR_STATE
+    assert L.hetd_steprk(h, 1e-4) == capi.HETD_ERR_STATE
+    assert L.hetd_steppc(h, 1e-4, ctypes.byref(d), ctypes.byref(d)) == capi.HETD_ERR_STATE
+    assert L.hetd_norm(h, ctypes.byref(d)) == capi.HETD_ERR_STATE
+    assert L.hetd_evolve(h, 1e-4, 1, None) == capi.HETD_ERR_STATE
+    assert L.hetd_set_uext(h, None) == capi.HETD_ERR_STATE
+    assert L.hetd_history_size(h) == -1
+    arr = (ctypes.c_void_p * 1)(h)
+    assert L.hetd_pool_evolve(arr, 1, 1e-4, 1) == capi.HETD_ERR_STATE
+    assert needle in L.hetd_last_error(h).decode()
+    assert L.hetd_finalize(h) == capi.HETD_OK
