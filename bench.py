@@ -278,8 +278,8 @@ def main():
                     help="untimed mixed-vs-FP64 accuracy run length (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the ensemble split across "
-              "GPUs is described in DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: the propagation step runs on one GPU; --gpus %d is not supported (independent "
+              "runs go one process per GPU, with no communication)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     from hetd import capi
     from hetd.initstate import woods_saxon_state
