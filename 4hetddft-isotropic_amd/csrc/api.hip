@@ -828,7 +828,10 @@ int hetd_build_potion(const char* selec, double r_cutoff, double umax, int32_t n
     const double d = rmax / (double)(np - 1);
     *npot = np;
     *delinter = d;
-    if (!potion) return HETD_OK;
+    if (!potion) {  // the query checks the name too
+        double v;
+        return hetd_vimp(selec, 0.0, r_cutoff, umax, &v);
+    }
     for (int i = 0; i < np; ++i) {
         const double r = (double)i * d;
         if (hetd_vimp(selec, r, r_cutoff, umax, &potion[i])) return HETD_ERR_ARG;

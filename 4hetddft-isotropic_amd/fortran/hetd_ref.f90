@@ -47,9 +47,8 @@ contains
     integer(c_int) :: rc
     character(kind=c_char, len=81) :: cs
     cs = trim(selec)//c_null_char
-    rc = hetd_init(p, hctx)
-    call hetd_check(rc, 'hetd_init')
-    ! potenimpini (potenimpini.f90:11-25)
+    ! potenimpini (potenimpini.f90:11-25); the table is built on the host, so
+    ! an unknown selec stops before the device is touched
     rc = hetd_build_potion(cs, r_cutoff, umax, max(p%nx, p%ny, p%nz), max(p%xmax, p%ymax, p%zmax), &
                            c_null_ptr, npot, delinter)
     if (rc /= HETD_OK) then
@@ -60,6 +59,8 @@ contains
     rc = hetd_build_potion(cs, r_cutoff, umax, max(p%nx, p%ny, p%nz), max(p%xmax, p%ymax, p%zmax), &
                            c_loc(potion), npot, delinter)
     call hetd_check(rc, 'hetd_build_potion')
+    rc = hetd_init(p, hctx)
+    call hetd_check(rc, 'hetd_init')
     rc = hetd_set_potion(hctx, potion, npot, delinter)
     call hetd_check(rc, 'hetd_set_potion')
   end subroutine hetd_setup

@@ -110,7 +110,7 @@ int hetd_vimp(const char *selec, double r, double r_cutoff, double umax, double 
 /* Interpolation table of potenimpini (potenimpini.f90:11-25):
  * npot = 100*max(nx,ny,nz)+1, delinter = 4*max(xmax,ymax,zmax)/(npot-1),
  * potion[i] = V(i*delinter).  potion must hold npot values (or be NULL to
- * query npot). */
+ * query npot).  HETD_ERR_ARG for an unknown selec, also on the query. */
 int hetd_build_potion(const char *selec, double r_cutoff, double umax,
                       int32_t nmax, double xmaxbox, double *potion,
                       int32_t *npot, double *delinter);

@@ -40,7 +40,9 @@ def test_driver_built():
 
 @pytest.mark.parametrize("over,msg", [(dict(nx=7), "NX is not correct"),
                                       (dict(deltat="1.d-4"), "Inconsistent deltat"),
-                                      (dict(mode=4), "This mode is still not programmed")])
+                                      (dict(mode=4), "This mode is still not programmed"),
+                                      (dict(selec="'Kr_He'"), "aquest potencial no esta definit"),
+                                      (dict(npd=4), "npd must be odd")])
 def test_driver_rejects_like_the_reference(tmp_path, over, msg):
     # these checks happen on the host before any device call
     cfg, _ = G.load("ot_xe_32")
