@@ -159,7 +159,7 @@ def test_fft_against_numpy(oracle_mod, shape):
     # c2r of a non-Hermitian half spectrum: canonical semantics (imaginary
     # parts of k=0 / Nyquist ignored along x), the same as numpy's irfftn
     G_ = F + 0.3 * (rng.standard_normal(F.shape) + 1j * rng.standard_normal(F.shape))
-    assert np.allclose(oracle_mod.fft_c2r(G_, shape[2]), np.fft.irfftn(G_, s=shape) * f.size,
+    assert np.allclose(oracle_mod.fft_c2r(G_, shape[2]), np.fft.irfftn(G_, s=shape, axes=(0, 1, 2)) * f.size,
                        rtol=1e-12, atol=1e-11)
 
 
