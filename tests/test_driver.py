@@ -52,8 +52,12 @@ def test_driver_rejects_like_the_reference(tmp_path, over, msg):
 
 
 @pytest.mark.gpu
-def test_driver_reproduces_reference_run(tmp_path):
-    cfg, d = G.load("ot_xe_32")
+@pytest.mark.parametrize("name", ["ot_xe_32", "otc_per_36x32x40", "ot_null_45x30x27"])
+def test_driver_reproduces_reference_run(tmp_path, name):
+    # the reference's own runs of three configurations: OT + solid + Xe with
+    # an absorbing tau (icon 13, npd 13), OTC periodic (icon 8, npd 9) and
+    # OT without an impurity on an odd 45x30x27 grid
+    cfg, d = G.load(name)
     write_mode2(os.path.join(tmp_path, "den.in"), d["psi0"], cfg["xmax"], cfg["ymax"], cfg["zmax"],
                 d["rimp0"], d["vimp0"])
     r = run_drive(str(tmp_path), namelist(cfg))
