@@ -48,6 +48,9 @@ class Context:
     def __init__(self, params, selec="null", r_cutoff=2.0, umax=0.0, potion=None):
         self.L = capi.load()
         self.p = params
+        if potion is None:  # an unknown selec raises before the device is touched
+            potion = build_potion(selec, r_cutoff, umax, max(params.nx, params.ny, params.nz),
+                                  max(params.xmax, params.ymax, params.zmax))
         h = C.c_void_p()
         rc = self.L.hetd_init(C.byref(params), C.byref(h))
         self.h = h
@@ -57,12 +60,17 @@ class Context:
             raise RuntimeError("hetd_init failed (%d): %s" % (rc, msg))
         self.shape = (params.nz, params.ny, params.nx)
         self.hshape = (params.nz, params.ny, params.nx // 2 + 1)
-        if potion is None:
-            potion = build_potion(selec, r_cutoff, umax, max(params.nx, params.ny, params.nz),
-                                  max(params.xmax, params.ymax, params.zmax))
         tab, d = potion
         self.potion, self.delinter = tab, d
         self._chk(self.L.hetd_set_potion(self.h, _p(tab), len(tab), d))
+
+    @staticmethod
+    def _host(a, dtype, shape, what):
+        # the C side reads exactly prod(shape) values: check before handing over
+        a = np.ascontiguousarray(a, dtype=dtype)
+        if a.shape != tuple(shape):
+            raise ValueError("%s: shape %s, expected %s (nz, ny, nx)" % (what, a.shape, tuple(shape)))
+        return a
 
     def _chk(self, rc):
         if rc != 0:
@@ -87,7 +95,7 @@ class Context:
 
     # -- reference call surface -------------------------------------------
     def set_state(self, psi, r, v):
-        a = np.ascontiguousarray(psi, dtype=np.complex128).view(np.float64)
+        a = self._host(psi, np.complex128, self.shape, "psi").view(np.float64)
         self._chk(self.L.hetd_set_state(self.h, _p(a), _p(_v3(r)), _p(_v3(v))))
 
     def init(self):
@@ -125,8 +133,7 @@ class Context:
         if uext is None:
             self._chk(self.L.hetd_set_uext(self.h, None))
             return
-        a = np.ascontiguousarray(uext, dtype=np.float64)
-        assert a.shape == self.shape
+        a = self._host(uext, np.float64, self.shape, "uext")
         self._chk(self.L.hetd_set_uext(self.h, _p(a)))
 
     def set_surface(self, lsurf3d, zsurf):
@@ -194,13 +201,13 @@ class Context:
         return r, v, a
 
     def fft_r2c(self, f):
-        f = np.ascontiguousarray(f, dtype=np.float64)
+        f = self._host(f, np.float64, self.shape, "f")
         out = np.zeros(self.hshape, dtype=np.complex128)
         self._chk(self.L.hetd_fft_r2c(self.h, _p(f), _p(out.view(np.float64))))
         return out
 
     def fft_c2r(self, F):
-        F = np.ascontiguousarray(F, dtype=np.complex128)
+        F = self._host(F, np.complex128, self.hshape, "F")
         out = np.zeros(self.shape)
         self._chk(self.L.hetd_fft_c2r(self.h, _p(F.view(np.float64)), _p(out)))
         return out
@@ -220,7 +227,7 @@ class Context:
 
     def deriv(self, f, id_, iv):
         cplx = np.iscomplexobj(f)
-        f = np.ascontiguousarray(f, dtype=np.complex128 if cplx else np.float64)
+        f = self._host(f, np.complex128 if cplx else np.float64, self.shape, "f")
         out = np.zeros_like(f)
         fv = f.view(np.float64) if cplx else f
         ov = out.view(np.float64) if cplx else out

@@ -83,3 +83,16 @@ def test_failed_init_leaves_an_inert_context(field, value, needle):
     assert L.hetd_pool_evolve(arr, 1, 1e-4, 1) == capi.HETD_ERR_STATE
     assert needle in L.hetd_last_error(h).decode()
     assert L.hetd_finalize(h) == capi.HETD_OK
+
+
+def test_python_binding_checks_before_the_device():
+    # both checks come before any HIP call
+    import numpy as np
+    from hetd import Context
+    p = capi.default_params(nx=16, ny=16, nz=16, xmax=3.2, ymax=3.2, zmax=3.2)
+    with pytest.raises(ValueError, match="unknown impurity potential"):
+        Context(p, selec="no_such_potential")
+    with pytest.raises(ValueError, match="expected"):
+        Context._host(np.zeros((16, 16, 15)), np.complex128, (16, 16, 16), "psi")
+    a = Context._host(np.zeros((16, 16, 16)), np.complex128, (16, 16, 16), "psi")
+    assert a.dtype == np.complex128 and a.flags.c_contiguous
