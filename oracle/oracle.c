@@ -727,8 +727,9 @@ int orc_steppc(orc_ctx *c, double dt, double *errHe, double *errimp) {
     den_from_psi(c);
     { int t = c->ioldh[1]; c->ioldh[1] = c->ioldh[0]; c->ioldh[0] = t; }
     double auxr[3];
-    /* the R predictor uses R(n-3) and the v history (steppc.f90:77-83) */
-    int r3 = c->ioldr[2] - 1;
+    /* the R predictor uses R(n-3) and the v history (steppc.f90:77-83),
+     * the corrector rimpold(ioldr(2)) (steppc.f90:148-149) */
+    int r2 = c->ioldr[1] - 1, r3 = c->ioldr[2] - 1;
     int v1 = c->ioldv[0] - 1, v2 = c->ioldv[1] - 1, v3 = c->ioldv[2] - 1;
     int a1 = c->iolda[0] - 1, a2 = c->iolda[1] - 1;
     for (int d = 0; d < 3; ++d)
