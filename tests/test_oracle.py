@@ -21,6 +21,18 @@ TOL = 1e-10  # relative, the FP64 parity bar
 
 # ---------------------------------------------------------------- golden
 
+
+def test_oracle_source_compiles(tmp_path):
+    """The oracle's C source as it stands compiles (the tests load a prebuilt
+    liboracle.so, which could hide a broken source)."""
+    import subprocess
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    r = subprocess.run(["gcc", "-O0", "-fPIC", "-std=c11", "-Wall", "-Wextra", "-Wno-unused-parameter",
+                        "-shared", "-o", str(tmp_path / "liboracle_check.so"), os.path.join(here, "oracle.c"),
+                        "-lm"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "warning" not in r.stderr, r.stderr
+
 @pytest.mark.parametrize("name", G.NAMES)
 def test_oracle_matches_reference_golden(oracle_mod, name):
     cfg, d = G.load(name)
