@@ -872,6 +872,8 @@ int hetd_init(const hetd_params* p, hetd_ctx** out) {
         if (!setup::radix_plan(nn[a], c->ax[a].rad, &c->ax[a].nst, kMaxStages))
             return fail(c, HETD_ERR_ARG, "grid edge %d has a prime factor above 5", nn[a]);
         c->ax[a].n = nn[a];
+        if (!stockham_plan_ok(c->ax[a]))
+            return fail(c, HETD_ERR_ARG, "grid edge %d: no FFT block shape fits", nn[a]);
     }
     HCHK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     if (const char* e = getenv("HETD_GRAPHS")) c->use_graphs = e[0] != '0';

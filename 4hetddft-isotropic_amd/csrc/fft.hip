@@ -202,6 +202,31 @@ static int pb_for(const FftAxis& ax, int nlines, int nthr) {
     return -1;
 }
 
+// Columns per y/z block: 8 (128-byte runs in FP64), fewer where a long line
+// would need more than 8 butterflies per thread (n = 750, 810, 864, 900, 960,
+// 972) or more LDS than a gfx950 workgroup holds (the FP64 z convolution,
+// two lines in LDS, above n = 640).  0: no width fits.
+static constexpr size_t kMaxLdsBytes = 160 * 1024;
+static int stockham_tx(const FftAxis& ax, int nl, size_t elt, int nthr) {
+    for (int tx = 8; tx >= 1; tx /= 2)
+        if (pb_for(ax, tx, nthr) > 0 && (size_t)nl * ax.n * tx * elt <= kMaxLdsBytes) return tx;
+    return 0;
+}
+
+static int stockham_ppb(int nx, long npairs) {
+    int ppb = (2048 + nx - 1) / nx;  // about 2048 complex points per block
+    if (ppb < 1) ppb = 1;
+    if (ppb > npairs) ppb = (int)npairs;
+    return ppb;
+}
+
+bool stockham_plan_ok(const FftAxis& ax) {
+    const int nthr = 256;
+    const int ppb = stockham_ppb(ax.n, 1L << 30);
+    const bool x_ok = pb_for(ax, ppb, nthr) > 0 && (size_t)ppb * ax.n * sizeof(cplx<double>) <= kMaxLdsBytes;
+    return x_ok && stockham_tx(ax, 2, sizeof(cplx<double>), nthr) > 0;
+}
+
 #define HETD_PB_SWITCH(PB, KERN, T, ...)                                                   \
     switch (PB) {                                                                          \
     case 1: hipLaunchKernelGGL((KERN<T, 1>), __VA_ARGS__); break;                          \
@@ -215,10 +240,8 @@ template <typename T>
 hipError_t launch_xpass(XPassArgs<T> a, hipStream_t st) {
     const int nthr = 256;
     const int nx = a.ax.n;
-    int ppb = (2048 + nx - 1) / nx;  // about 2048 complex points per block
-    if (ppb < 1) ppb = 1;
     const long npairs = (a.nlines + 1) / 2;
-    if (ppb > npairs) ppb = (int)npairs;
+    const int ppb = stockham_ppb(nx, npairs);
     a.ppb = ppb;
     const int pb = pb_for(a.ax, ppb, nthr);
     const size_t lds = (size_t)ppb * nx * sizeof(cplx<T>);
@@ -230,7 +253,8 @@ hipError_t launch_xpass(XPassArgs<T> a, hipStream_t st) {
 template <typename T>
 hipError_t launch_ypass(YPassArgs<T> a, int nz, int nbuf, hipStream_t st) {
     const int nthr = 256;
-    a.tx = 8;
+    a.tx = stockham_tx(a.ax, 1, sizeof(cplx<T>), nthr);
+    if (a.tx == 0) return hipErrorInvalidValue;
     const int pb = pb_for(a.ax, a.tx, nthr);
     const size_t lds = (size_t)a.ax.n * a.tx * sizeof(cplx<T>);
     dim3 grid((a.nxh + a.tx - 1) / a.tx, nz, nbuf);
@@ -241,9 +265,10 @@ hipError_t launch_ypass(YPassArgs<T> a, int nz, int nbuf, hipStream_t st) {
 template <typename T>
 hipError_t launch_zpass(ZPassArgs<T> a, int nbuf, hipStream_t st) {
     const int nthr = 256;
-    a.tx = 8;
-    const int pb = pb_for(a.ax, a.tx, nthr);
     const int nl = (a.mode == ZMODE_CONV) ? 2 : 1;
+    a.tx = stockham_tx(a.ax, nl, sizeof(cplx<T>), nthr);
+    if (a.tx == 0) return hipErrorInvalidValue;
+    const int pb = pb_for(a.ax, a.tx, nthr);
     const size_t lds = (size_t)nl * a.ax.n * a.tx * sizeof(cplx<T>);
     dim3 grid((a.nxh + a.tx - 1) / a.tx, a.ny, nbuf);
     HETD_PB_SWITCH(pb, k_zpass, T, grid, dim3(nthr), lds, st, a);

@@ -86,6 +86,9 @@ template <typename T> struct ZPassArgs {
 template <typename T> hipError_t launch_xpass(XPassArgs<T> a, hipStream_t st);
 template <typename T> hipError_t launch_ypass(YPassArgs<T> a, int nz, int nbuf, hipStream_t st);
 template <typename T> hipError_t launch_zpass(ZPassArgs<T> a, int nbuf, hipStream_t st);
+// the LDS Stockham passes can transform lines of this axis (block shapes that
+// fit the thread count and the LDS, in FP64 and FP32)
+bool stockham_plan_ok(const FftAxis& ax);
 // register-resident passes for axis lengths 16, 64, 256 (fft_reg.hip)
 bool reg_fft_supported(int n);
 template <typename T> hipError_t launch_xreg(XPassArgs<T> a, hipStream_t st);

@@ -28,7 +28,10 @@ def make_pair(oracle_mod, p, selec="null", r_cutoff=2.0, umax=0.0):
 @pytest.mark.parametrize("shape", [(8, 6, 10), (27, 10, 12), (16, 16, 16), (5, 9, 45), (20, 12, 36),
                                    (32, 24, 64), (40, 30, 50), (16, 64, 256), (256, 16, 64), (64, 256, 16),
                                    (15, 9, 256), (384, 6, 10), (6, 384, 12), (8, 6, 384), (162, 20, 216),
-                                   (12, 288, 320), (192, 180, 150)])
+                                   (12, 288, 320), (192, 180, 150),
+                                   # LDS Stockham passes on long lines: narrower column
+                                   # blocks where 8 columns would not fit (750, 810, 960)
+                                   (4, 750, 6), (810, 4, 6), (6, 4, 972), (960, 5, 4)])
 def test_fft_matches_numpy_and_oracle(oracle_mod, shape):
     say("fft", shape)
     nz, ny, nx = shape
@@ -158,6 +161,35 @@ def test_fine_grained_calls_match_oracle(oracle_mod):
         assert G.rel(ctx.field(capi.F_PC), orc.field(capi.F_PC)) < 1e-8
         for x, y in zip(ctx.imp(), orc.imp()):
             assert np.abs(x - y).max() <= TOL * max(np.abs(y).max(), 1e-300)
+
+
+@pytest.mark.parametrize("shape,core", [((720, 4, 6), "OT"), ((750, 6, 4), "OTC"), ((4, 864, 6), "OT")])
+def test_poten_long_axes_match_oracle(oracle_mod, shape, core):
+    """The mean field on grids whose long axis takes the LDS Stockham passes
+    with narrowed column blocks: the z convolution above 640 points in FP64,
+    and the lengths that need them for the butterfly count (750, 864)."""
+    say("poten long axis", shape, core)
+    nz, ny, nx = shape
+    h = 0.5
+    p = capi.default_params(nx=nx, ny=ny, nz=nz, xmax=nx * h / 2, ymax=ny * h / 2, zmax=nz * h / 2, npd=9,
+                            icon=8, core=core, lsolid=0 if core == "OTC" else 1)
+    rng = np.random.default_rng(4)
+    z = np.arange(nz)[:, None, None]
+    rho = 0.02 + 0.004 * np.cos(2 * np.pi * 3 * z / nz) + 0.001 * rng.random(shape)
+    psi = np.sqrt(rho) * np.exp(1j * rng.uniform(-0.1, 0.1, shape))
+    ctx, orc = make_pair(oracle_mod, p, "Xe_He", 2.55, 2793.9365463)
+    for c in (ctx, orc):
+        c.set_state(psi, (0.2, -0.1, 3.0), (0.0, 0.0, 1.0))
+        c.init()
+    for f in (capi.F_POT4, capi.F_DENCG, capi.F_DELJ4):
+        assert G.rel(ctx.field(f), orc.field(f)) < TOL, f
+    e1, e2 = ctx.energy(), orc.energy()
+    for k in e2:
+        assert abs(e1[k] - e2[k]) <= TOL * max(abs(e2[k]), 1.0), k
+    t1 = ctx.evolve(G.DT, 4)
+    t2 = orc.evolve(G.DT, 4)
+    assert np.abs(t1[:, 2:] - t2[:, 2:]).max() <= TOL * np.abs(t2[:, 2:]).max()
+    assert G.rel(ctx.field(capi.F_PSI), orc.field(capi.F_PSI)) < TOL
 
 
 @pytest.mark.parametrize("shape,npd,icon,core", [((48, 40, 36), 13, 13, "OT"), ((30, 32, 45), 9, 8, "OT"),
