@@ -192,6 +192,27 @@ def test_poten_long_axes_match_oracle(oracle_mod, shape, core):
     assert G.rel(ctx.field(capi.F_PSI), orc.field(capi.F_PSI)) < TOL
 
 
+@pytest.mark.parametrize("shape", [(750, 6, 4), (4, 960, 6)])
+def test_mixed_poten_long_axes_tracks_fp64(shape):
+    """The FP32 Stockham passes on the same long lines: the mixed mean field
+    against the FP64 one (FP32 rounding, 1e-5 relative)."""
+    say("mixed poten long axis", shape)
+    nz, ny, nx = shape
+    h = 0.5
+    kw = dict(nx=nx, ny=ny, nz=nz, xmax=nx * h / 2, ymax=ny * h / 2, zmax=nz * h / 2, npd=9, icon=8)
+    rng = np.random.default_rng(5)
+    rho = 0.02 + 0.002 * rng.random(shape)
+    psi = np.sqrt(rho) * np.exp(1j * rng.uniform(-0.1, 0.1, shape))
+    pot = {}
+    for prec in (capi.HETD_FP64, capi.HETD_MIXED):
+        with Context(capi.default_params(precision=prec, **kw), selec="Xe_He", r_cutoff=2.55,
+                     umax=2793.9365463) as ctx:
+            ctx.set_state(psi, (0.0, 0.0, 1.0), (0.0, 0.0, 0.0))
+            ctx.init()
+            pot[prec] = ctx.field(capi.F_POT4)
+    assert G.rel(pot[capi.HETD_MIXED], pot[capi.HETD_FP64]) < 1e-5
+
+
 @pytest.mark.parametrize("shape,npd,icon,core", [((48, 40, 36), 13, 13, "OT"), ((30, 32, 45), 9, 8, "OT"),
                                                  ((36, 36, 36), 13, 18, "OTC"), ((64, 64, 64), 13, 13, "OT"),
                                                  ((16, 64, 64), 9, 8, "OTC")])
