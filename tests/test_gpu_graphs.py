@@ -147,6 +147,38 @@ def test_pingpong_update_variant_is_bitwise(shape, prec, icon, tile):
     b.close()
 
 
+@pytest.mark.parametrize("shape,prec,icon", [((36, 40, 48), capi.HETD_FP64, 13), ((32, 24, 40), capi.HETD_MIXED, 8)])
+def test_force_energy_tile_variants_agree(shape, prec, icon):
+    """HETD_TILE_FORCE / HETD_TILE_ENERGY = 0..5 (the tile shapes of the
+    standalone force and of the energy sums): the sums are taken over other
+    tiles, so results agree to rounding (1e-12), not bit for bit."""
+    say("force/energy tiles", shape, prec, icon)
+    ref = make(shape, prec, icon, graphs=False)
+    ref.evolve(G.DT, 6, traj=False)
+    e_ref = ref.energy()
+    r_ref = ref.imp()
+    psi_ref = ref.field(capi.F_PSI)
+    ref.close()
+    for v in "012345":
+        os.environ["HETD_TILE_FORCE"] = v
+        os.environ["HETD_TILE_ENERGY"] = v
+        try:
+            c = make(shape, prec, icon, graphs=False)
+            c.evolve(G.DT, 6, traj=False)
+            e = c.energy()
+            r = c.imp()
+            psi = c.field(capi.F_PSI)
+            c.close()
+        finally:
+            del os.environ["HETD_TILE_FORCE"]
+            del os.environ["HETD_TILE_ENERGY"]
+        for k in e_ref:
+            assert abs(e[k] - e_ref[k]) <= 1e-12 * max(abs(e_ref[k]), 1.0), (v, k)
+        for x, y in zip(r, r_ref):
+            assert np.abs(x - y).max() <= 1e-12 * max(np.abs(y).max(), 1e-300), v
+        assert G.rel(psi, psi_ref) < 1e-12, v
+
+
 @pytest.mark.parametrize("shape,prec,icon", [((32, 32, 64), capi.HETD_FP64, 13), ((30, 32, 16), capi.HETD_MIXED, 8)])
 def test_autotune_leaves_state_and_results_unchanged(shape, prec, icon):
     """hetd_autotune times variants that compute the same bits and restores
