@@ -108,6 +108,28 @@ def test_poten_energy_force_match_golden(oracle_mod, name):
 
 
 @pytest.mark.parametrize("name", G.NAMES)
+def test_mixed_fields_track_golden(name):
+    """Every field copy of the mixed path (FP32 fields widened on the way out)
+    against the reference's FP64 fields: FP32 rounding, 1e-5 relative."""
+    say("golden init mixed", name)
+    cfg, d = G.load(name)
+    p = G.params_for(cfg, capi, precision=capi.HETD_MIXED)
+    with Context(p, selec=cfg["selec"], r_cutoff=cfg["r_cutoff"], umax=cfg["umax"]) as ctx:
+        ctx.set_state(d["psi0"], d["rimp0"], d["vimp0"])
+        ctx.init()
+        for key, fid in (("pot4", capi.F_POT4), ("dencg", capi.F_DENCG), ("delj4", capi.F_DELJ4),
+                         ("ccor", capi.F_CCOR), ("ualphas", capi.F_UALPHAS), ("denalf", capi.F_DENALF)):
+            if "init_%s_planes" % key not in d or (key == "ccor" and cfg["core"] == "OTC"):
+                continue
+            assert G.rel(G.planes(ctx.field(fid)), d["init_%s_planes" % key]) < 1e-5, key
+        if cfg["core"] != "OTC":
+            assert G.rel(G.planes(ctx.field(capi.F_FDEN)), d["init_fden_planes"]) < 1e-5
+        e = ctx.energy()
+        for i, k in enumerate(("etot4", "ekin4", "elj4", "ecor4", "esolid", "ealphas", "ekinx", "eHeX", "etot")):
+            assert abs(e[k] - d["init_energy"][i]) <= 1e-5 * max(abs(d["init_energy"][i]), 1.0), k
+
+
+@pytest.mark.parametrize("name", G.NAMES)
 def test_evolve_matches_golden_trajectory(oracle_mod, name):
     say("golden evolve", name)
     cfg, d = G.load(name)
