@@ -240,7 +240,10 @@ def test_mixed_poten_long_axes_tracks_fp64(shape):
                                                  ((16, 64, 64), 9, 8, "OTC"),
                                                  # the generic stencil kernels (npd other than 9, 13)
                                                  ((24, 20, 30), 7, 13, "OT"), ((20, 24, 18), 11, 18, "OTC"),
-                                                 ((18, 20, 24), 3, 8, "OT"), ((20, 18, 16), 25, 13, "OT")])
+                                                 ((18, 20, 24), 3, 8, "OT"), ((20, 18, 16), 25, 13, "OT"),
+                                                 # an axis shorter than the stencil: the reference's
+                                                 # derivative along it is zero (tiled kernels)
+                                                 ((8, 40, 36), 13, 13, "OT"), ((36, 8, 40), 9, 8, "OT")])
 def test_evolve_matches_oracle_other_grids(oracle_mod, shape, npd, icon, core):
     say("oracle evolve", shape, npd, icon, core)
     from hetd.initstate import woods_saxon_state
