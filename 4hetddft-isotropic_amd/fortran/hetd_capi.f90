@@ -169,6 +169,11 @@ module hetd_capi
       integer(c_int32_t), value :: which
       type(c_ptr), value :: host
     end function
+    integer(c_int) function hetd_deriv(ctx, id, iv, is_complex, f, df) bind(C, name='hetd_deriv')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx, f, df
+      integer(c_int32_t), value :: id, iv, is_complex
+    end function
     integer(c_int) function hetd_get_imp(ctx, r, v, a) bind(C, name='hetd_get_imp')
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx

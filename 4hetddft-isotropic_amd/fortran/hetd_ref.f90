@@ -8,6 +8,7 @@
 !   steprk   steprk.f90:1-153      steppc    steppc.f90:1-178
 !   energy   energy.f90:12-144     (energies as in module energies,
 !                                   modules.f90:41-52)
+!   derivnD  init_deriv_parallel.f90:2-48 (the 3-D real and complex forms)
 ! A failing call STOPs with the library's message, as the reference STOPs.
 !-----------------------------------------------------------------------------
 module hetd_ref
@@ -16,6 +17,15 @@ module hetd_ref
   type(c_ptr), save :: hctx = c_null_ptr
   real(8), save :: etot = 0, etot4 = 0, ekin4 = 0, ekinx = 0, elj4 = 0, ealphas = 0
   real(8), save :: ecor4 = 0, eimpu = 0, eHeX = 0, esolid = 0
+  ! grid, spacing and boundary rule of the context, for derivnD's checks
+  integer, save :: hgrid(3) = 0, hicon = 0
+  real(8), save :: hspace(3) = 0
+
+  ! derivnD (init_deriv_parallel.f90:2-48): the id-th derivative along
+  ! axis iv with the context's npd-point stencil
+  interface derivnD
+    module procedure deriv3d_p, deriv3d_pc
+  end interface derivnD
 
 contains
 
@@ -61,6 +71,9 @@ contains
     call hetd_check(rc, 'hetd_build_potion')
     rc = hetd_init(p, hctx)
     call hetd_check(rc, 'hetd_init')
+    hgrid = [p%nx, p%ny, p%nz]
+    hicon = p%icon
+    hspace = 2.0d0*abs([p%xmax, p%ymax, p%zmax])/hgrid  ! DFT4He3d.f90:274
     rc = hetd_set_potion(hctx, potion, npot, delinter)
     call hetd_check(rc, 'hetd_set_potion')
   end subroutine hetd_setup
@@ -97,5 +110,35 @@ contains
     etot = e%etot; etot4 = e%etot4; ekin4 = e%ekin4; ekinx = e%ekinx; elj4 = e%elj4
     ealphas = e%ealphas; ecor4 = e%ecor4; eimpu = e%eimpu; eHeX = e%eHeX; esolid = e%esolid
   end subroutine energy
+
+  ! The library differentiates with the stencil, spacing and boundary rule
+  ! of its context: a call for another grid, spacing or icon stops.
+  subroutine deriv_args(id, nn, h, iv, icon)
+    integer(4), intent(in) :: id, nn(3), iv, icon
+    real(8), intent(in) :: h
+    if (.not. c_associated(hctx)) error stop 'derivnD: hetd_setup has not been called'
+    if (id < 1 .or. id > 2 .or. iv < 1 .or. iv > 3) error stop 'derivnD: id must be 1 or 2, iv 1 to 3'
+    if (any(nn /= hgrid)) error stop 'derivnD: the grid differs from the context''s'
+    if (icon /= hicon) error stop 'derivnD: icon differs from the context''s'
+    if (abs(h - hspace(iv)) > 1.0d-12*hspace(iv)) error stop 'derivnD: h differs from the context''s spacing'
+  end subroutine deriv_args
+
+  subroutine deriv3d_p(id, nn, h, iv, f, df, icon)
+    integer(4), intent(in) :: id, nn(3), iv, icon
+    real(8), intent(in) :: h
+    real(8), intent(in), target :: f(nn(1), nn(2), nn(3))
+    real(8), intent(out), target :: df(nn(1), nn(2), nn(3))
+    call deriv_args(id, nn, h, iv, icon)
+    call hetd_check(hetd_deriv(hctx, id, iv, 0, c_loc(f), c_loc(df)), 'derivnD')
+  end subroutine deriv3d_p
+
+  subroutine deriv3d_pc(id, nn, h, iv, f, df, icon)
+    integer(4), intent(in) :: id, nn(3), iv, icon
+    real(8), intent(in) :: h
+    complex(8), intent(in), target :: f(nn(1), nn(2), nn(3))
+    complex(8), intent(out), target :: df(nn(1), nn(2), nn(3))
+    call deriv_args(id, nn, h, iv, icon)
+    call hetd_check(hetd_deriv(hctx, id, iv, 1, c_loc(f), c_loc(df)), 'derivnD')
+  end subroutine deriv3d_pc
 
 end module hetd_ref

@@ -12,6 +12,7 @@ from hetd.initstate import read_mode2, write_mode2
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVE = os.path.join(ROOT, "4hetddft-isotropic_amd", "bin", "hetd_drive")
+REF_LOOP = os.path.join(ROOT, "4hetddft-isotropic_amd", "bin", "hetd_ref_loop")
 
 
 def namelist(cfg, **over):
@@ -36,6 +37,65 @@ def run_drive(tmp_path, text):
 
 def test_driver_built():
     assert os.path.exists(DRIVE), "bin/hetd_drive not built (run __graft_entry__.build())"
+    assert os.path.exists(REF_LOOP), "bin/hetd_ref_loop not built (run __graft_entry__.build())"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["ot_xe_32", "otc_per_36x32x40"])
+def test_reference_loop_through_hetd_ref(tmp_path, oracle_mod, name):
+    """The reference's time loop written routine by routine against module
+    hetd_ref (steprk, steppc, potenimp, poten, forceimp, energy, derivnD):
+    the reference's own trajectory and final state, the oracle's energies and
+    derivatives."""
+    cfg, d = G.load(name)
+    nz, ny, nx = d["psi0"].shape
+    niter = 8
+    state = os.path.join(tmp_path, "state.bin")
+    with open(state, "wb") as f:
+        # Fortran order psi(nx,ny,nz) is the C order of the (nz,ny,nx) array
+        np.ascontiguousarray(d["psi0"], dtype=np.complex128).tofile(f)
+        np.asarray(d["rimp0"], dtype=np.float64).tofile(f)
+        np.asarray(d["vimp0"], dtype=np.float64).tofile(f)
+    text = ("&loop nx=%d, ny=%d, nz=%d, xmax=%r, ymax=%r, zmax=%r, npd=%d, icon=%d, core4='%s',\n"
+            " lsolid=%s, selec='%s', r_cutoff=%r, umax=%r, mimpur=131.29d0, deltat=%r, niter=%d,\n"
+            " tzmean=%r, statein='%s', prefix='%s/' /\n") % (
+        nx, ny, nz, float(cfg["xmax"]), float(cfg["ymax"]), float(cfg["zmax"]), cfg["npd"], cfg["icon"],
+        cfg["core"], ".true." if cfg["lsolid"] else ".false.", cfg["selec"], float(cfg["r_cutoff"]),
+        float(cfg["umax"]), float(G.DT), niter, float(cfg["tzmean"]), state, str(tmp_path))
+    r = subprocess.run([REF_LOOP], input=text, cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr
+    traj = np.fromfile(os.path.join(tmp_path, "traj.bin")).reshape(niter, 11)
+    st = d["steps"][:niter]
+    # golden columns: iter, errHe, errimp, rimp(3), vimp(3), aimp(3)
+    assert np.abs(traj[:, 2:] - st[:, 3:]).max() <= 1e-10 * np.abs(st[:, 3:]).max()
+    assert np.abs(traj[3:, 0] - st[3:, 1]).max() <= 1e-8 * np.abs(st[3:, 1]).max() + 1e-15
+
+    from hetd import capi
+    p = G.params_for(cfg, capi)
+    orc = oracle_mod.OracleCtx(p, selec=cfg["selec"], r_cutoff=cfg["r_cutoff"], umax=cfg["umax"])
+    orc.set_state(d["psi0"], d["rimp0"], d["vimp0"])
+    orc.init()
+    orc.evolve(G.DT, niter)
+    psi = np.fromfile(os.path.join(tmp_path, "psi.bin"), dtype=np.complex128).reshape(nz, ny, nx)
+    assert G.rel(psi, orc.field(capi.F_PSI)) < 1e-10
+    e = np.fromfile(os.path.join(tmp_path, "energy.bin"))
+    eo = orc.energy()
+    for k, v in zip(("etot4", "ekin4", "elj4", "ecor4", "esolid", "ealphas", "ekinx", "eHeX", "eimpu", "etot"), e):
+        assert abs(v - eo[k]) <= 1e-10 * max(abs(eo[k]), 1.0), k
+    der = np.fromfile(os.path.join(tmp_path, "deriv.bin"))
+    n = nx * ny * nz
+    h = [2.0 * abs(cfg[k]) / m for k, m in (("xmax", nx), ("ymax", ny), ("zmax", nz))]
+    den = np.abs(psi) ** 2
+    for iv in (1, 2, 3):
+        got = der[(iv - 1) * n:iv * n].reshape(nz, ny, nx)
+        want = oracle_mod.deriv(den, cfg["npd"], cfg["icon"], 1, iv, h[iv - 1])
+        assert G.rel(got, want) < 1e-12, ("d den", iv)
+    off = 3 * n
+    for iv in (1, 2, 3):
+        got = der[off + 2 * (iv - 1) * n:off + 2 * iv * n].view(np.complex128).reshape(nz, ny, nx)
+        want = oracle_mod.deriv(psi, cfg["npd"], cfg["icon"], 2, iv, h[iv - 1])
+        assert G.rel(got, want) < 1e-12, ("d2 psi", iv)
 
 
 @pytest.mark.parametrize("over,msg", [(dict(nx=7), "NX is not correct"),
