@@ -213,16 +213,21 @@ static int stockham_tx(const FftAxis& ax, int nl, size_t elt, int nthr) {
     return 0;
 }
 
-static int stockham_ppb(int nx, long npairs) {
-    int ppb = (2048 + nx - 1) / nx;  // about 2048 complex points per block
+// Line pairs per x block: about 2048 complex points, fewer where that would
+// take more than 4 butterflies per thread (lines with a radix-2 stage: nx =
+// 6, 10, 18, 54, 250, 270, 450, 486, 750, 810); the FP64 kernel built for 8
+// spills registers.
+static int stockham_ppb(const FftAxis& ax, long npairs, int nthr) {
+    int ppb = (2048 + ax.n - 1) / ax.n;
     if (ppb < 1) ppb = 1;
     if (ppb > npairs) ppb = (int)npairs;
+    while (ppb > 1 && pb_for(ax, ppb, nthr) > 4) --ppb;
     return ppb;
 }
 
 bool stockham_plan_ok(const FftAxis& ax) {
     const int nthr = 256;
-    const int ppb = stockham_ppb(ax.n, 1L << 30);
+    const int ppb = stockham_ppb(ax, 1L << 30, nthr);
     const bool x_ok = pb_for(ax, ppb, nthr) > 0 && (size_t)ppb * ax.n * sizeof(cplx<double>) <= kMaxLdsBytes;
     return x_ok && stockham_tx(ax, 2, sizeof(cplx<double>), nthr) > 0;
 }
@@ -241,7 +246,7 @@ hipError_t launch_xpass(XPassArgs<T> a, hipStream_t st) {
     const int nthr = 256;
     const int nx = a.ax.n;
     const long npairs = (a.nlines + 1) / 2;
-    const int ppb = stockham_ppb(nx, npairs);
+    const int ppb = stockham_ppb(a.ax, npairs, nthr);
     a.ppb = ppb;
     const int pb = pb_for(a.ax, ppb, nthr);
     const size_t lds = (size_t)ppb * nx * sizeof(cplx<T>);

@@ -31,7 +31,9 @@ def make_pair(oracle_mod, p, selec="null", r_cutoff=2.0, umax=0.0):
                                    (12, 288, 320), (192, 180, 150),
                                    # LDS Stockham passes on long lines: narrower column
                                    # blocks where 8 columns would not fit (750, 810, 960)
-                                   (4, 750, 6), (810, 4, 6), (6, 4, 972), (960, 5, 4)])
+                                   (4, 750, 6), (810, 4, 6), (6, 4, 972), (960, 5, 4),
+                                   # x lines with a radix-2 stage: fewer pairs per block
+                                   (6, 5, 250), (4, 3, 810), (3, 4, 54)])
 def test_fft_matches_numpy_and_oracle(oracle_mod, shape):
     say("fft", shape)
     nz, ny, nx = shape
