@@ -1583,17 +1583,11 @@ int hetd_get_field(hetd_ctx* c, int32_t which, double* host) {
         HCHK(c, launch_uimp_field(c->grid, c->imp, c->tmp, c->st));
         src = c->tmp;
         break;
-    case HETD_F_PENALTY: {
+    case HETD_F_PENALTY:
         if (!c->p.lsolid) return fail(c, HETD_ERR_ARG, "no penalty without lsolid");
-        std::vector<double> d(c->n);
-        HCHK(c, hipMemcpyAsync(d.data(), c->den, nR, hipMemcpyDeviceToHost, c->st));
-        HCHK(c, hipStreamSynchronize(c->st));
-        for (long i = 0; i < c->n; ++i) {
-            const double t = std::tanh(c->p.beta_solid * (d[i] - c->p.den_m));
-            host[i] = c->p.c_solid * (1.0 + t + c->p.beta_solid * d[i] * (1.0 - t * t));
-        }
-        return HETD_OK;
-    }
+        HCHK(c, launch_penalty_field(c->den, c->p.c_solid, c->p.beta_solid, c->p.den_m, c->tmp, c->n, c->st));
+        src = c->tmp;
+        break;
     case HETD_F_UALPHAS:
     case HETD_F_DENALF:
         if (!c->otc) return fail(c, HETD_ERR_ARG, "alpha_s fields exist only with core OTC");

@@ -269,6 +269,8 @@ hipError_t launch_den32_from_psi(const double2* psi, float* den, long n, hipStre
 hipError_t launch_to_double(const float* a, double* out, long n, hipStream_t st);
 hipError_t launch_to_float(const double* a, float* out, long n, hipStream_t st);
 hipError_t launch_uimp_field(const Grid& g, const ImpState* imp, double* out, hipStream_t st);
+hipError_t launch_penalty_field(const double* den, double csol, double beta, double den_m, double* out, long n,
+                                hipStream_t st);
 // OTC helpers (term_alfa.f90)
 hipError_t launch_falfs(const double* denalf, double den0s, double* falfs, long n, hipStream_t st);
 hipError_t launch_mul(const double* a, const double* b, double* out, long n, hipStream_t st);

@@ -267,6 +267,18 @@ hipError_t launch_uimp_field(const Grid& g, const ImpState* imp, double* out, hi
     return hipGetLastError();
 }
 
+// the solid penalty field of poten.f90:80-94, with the device function the
+// mean field adds (a field copy for callers; the step never stores it)
+__global__ void k_penalty_field(const double* den, double csol, double beta, double den_m, double* out, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = solid_penalty<double>(den[i], csol, beta, den_m);
+}
+hipError_t launch_penalty_field(const double* den, double csol, double beta, double den_m, double* out, long n,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(k_penalty_field, grid1(n), dim3(256), 0, st, den, csol, beta, den_m, out, n);
+    return hipGetLastError();
+}
+
 __global__ void k_falfs(const double* denalf, double den0s, double* falfs, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) falfs[i] = 1.0 - denalf[i] / den0s;

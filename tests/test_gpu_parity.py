@@ -94,6 +94,15 @@ def test_poten_energy_force_match_golden(oracle_mod, name):
         if key == "ccor" and cfg["core"] == "OTC":
             continue  # term_alfa reuses sto1 (term_alfa.f90:83-86)
         got = ctx.field(fid)
+        if key == "penalty":
+            # The reference forms 1 + tanh(x) with tanh near -1 (densities far
+            # below den_m), which keeps ~4 digits of a ~1e-8 K penalty; the
+            # library uses the cancellation-free exp form of its mean field.
+            # Compared on the scale at which it enters pot4.
+            scale = np.abs(d["init_pot4_planes"]).max()
+            assert np.abs(G.planes(got) - d["init_penalty_planes"]).max() < TOL * scale, key
+            assert np.abs(got - orc.field(fid)).max() < TOL * scale, key
+            continue
         assert G.rel(G.planes(got), d["init_%s_planes" % key]) < TOL, key
         assert G.rel(got, orc.field(fid)) < TOL, key
     if cfg["core"] != "OTC":
