@@ -35,6 +35,19 @@ def run_drive(tmp_path, text):
         return subprocess.run([DRIVE], stdin=fin, cwd=tmp_path, capture_output=True, text=True, timeout=300)
 
 
+def test_fortran_sources_compile(tmp_path):
+    """The Fortran host sources as they stand compile (the tests run prebuilt
+    binaries, which could hide a broken source)."""
+    fdir = os.path.join(ROOT, "4hetddft-isotropic_amd", "fortran")
+    fc = "/opt/rocm/llvm/bin/flang"
+    if not os.path.exists(fc):
+        pytest.skip("flang not found")
+    for src in ("hetd_capi.f90", "hetd_ref.f90", "hetd_drive.f90", "hetd_ref_loop.f90"):
+        r = subprocess.run([fc, "-O0", "-module-dir", str(tmp_path), "-c", os.path.join(fdir, src), "-o",
+                            str(tmp_path / (src + ".o"))], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, (src, r.stderr)
+
+
 def test_driver_built():
     assert os.path.exists(DRIVE), "bin/hetd_drive not built (run __graft_entry__.build())"
     assert os.path.exists(REF_LOOP), "bin/hetd_ref_loop not built (run __graft_entry__.build())"
