@@ -50,7 +50,8 @@ extern "C" {
 #define HETD_F_DYDEN 7
 #define HETD_F_DZDEN 8
 #define HETD_F_UIMP 9     /* impurity field u_imp (potenimp.f90:71-95) */
-#define HETD_F_PENALTY 10 /* solid penalty (poten.f90:80-83) */
+#define HETD_F_PENALTY 10 /* solid penalty (poten.f90:80-83), in the cancellation-free
+                             form the mean field uses (1 + tanh x as 2e^{2x}/(1+e^{2x})) */
 #define HETD_F_UALPHAS 11 /* alpha_s field (term_alfa.f90) */
 #define HETD_F_DENALF 12  /* Gaussian-smoothed density (poten.f90:96-101) */
 #define HETD_F_FVLJ 13    /* half spectrum: LJ form factor (fforma.f90) */
