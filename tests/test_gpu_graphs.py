@@ -98,8 +98,20 @@ def test_history_restart_is_bitwise(shape, prec, icon):
         a.evolve(G.DT, n, traj=False)
         b.evolve(G.DT, n, traj=False)
         same_state(a, b)
+    # rejected blobs: truncated, not a history, another grid; the state is kept
     with pytest.raises(RuntimeError):
         b.load_history(blob[:-8])
+    bad = bytearray(blob)
+    bad[0:8] = b"NOT_HIST"
+    with pytest.raises(RuntimeError, match="not a history"):
+        b.load_history(bytes(bad))
+    c = make((16, 16, 16), prec, icon, graphs=False)
+    with pytest.raises(RuntimeError, match="another grid"):
+        c.load_history(blob)
+    c.close()
+    a.evolve(G.DT, 3, traj=False)
+    b.evolve(G.DT, 3, traj=False)
+    same_state(a, b)
     a.close()
     b.close()
 
