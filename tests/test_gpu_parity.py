@@ -415,6 +415,38 @@ def test_guard_reports_nonpositive_density():
             ctx.steprk(G.DT)
 
 
+def test_requests_the_context_cannot_serve_are_rejected():
+    """Field copies that do not exist for the configuration, an unknown field,
+    calls before the state, and a table of another size: each returns an
+    error with a message and leaves the context usable."""
+    say("rejections")
+    p = capi.default_params(nx=16, ny=16, nz=16, xmax=4.0, ymax=4.0, zmax=4.0, npd=9, lsolid=0,
+                            core=capi.HETD_CORE_OTC, icon=8)
+    with Context(p, selec="Xe_He", r_cutoff=2.55, umax=2793.9365463) as ctx:
+        with pytest.raises(RuntimeError, match="no state"):
+            ctx.steprk(G.DT)
+        psi = np.full((16, 16, 16), 0.1 + 0j)
+        ctx.set_state(psi, (0, 0, 0), (0, 0, 0))
+        ctx.init()
+        with pytest.raises(RuntimeError, match="no penalty without lsolid"):
+            ctx.field(capi.F_PENALTY)
+        with pytest.raises(RuntimeError, match="FDEN not kept with OTC"):
+            ctx.field(capi.F_FDEN)
+        with pytest.raises(RuntimeError, match="unknown field"):
+            ctx.field(99)
+        tab = np.zeros(len(ctx.potion) + 1)
+        with pytest.raises(RuntimeError, match="table size changed"):
+            ctx._chk(ctx.L.hetd_set_potion(ctx.h, tab.ctypes.data_as(capi.C.POINTER(capi.C.c_double)),
+                                           len(tab), ctx.delinter))
+        ctx.evolve(G.DT, 4)
+        assert np.isfinite(ctx.energy()["etot"])
+    p2 = capi.default_params(nx=16, ny=16, nz=16, xmax=4.0, ymax=4.0, zmax=4.0, npd=9)
+    with Context(p2) as ctx:
+        ctx.set_state(psi, (0, 0, 0), (0, 0, 0))
+        with pytest.raises(RuntimeError, match="only with core OTC"):
+            ctx.field(capi.F_UALPHAS)
+
+
 # ------------------------------------------------------ mixed precision
 
 @pytest.mark.parametrize("shape", [(64, 64, 64), (36, 40, 48)])
