@@ -147,6 +147,36 @@ def test_driver_reproduces_reference_run(tmp_path, name):
 
 
 @pytest.mark.gpu
+def test_driver_lsurf_key_adds_the_surface_energy(tmp_path):
+    """lsurf (DFT4He3d.f90:758-784): the TiO2 surface potential enters the
+    helium energy as sum uext*rho*dV and leaves the dynamics alone."""
+    import re
+    from hetd import vsurf
+    cfg, d = G.load("ot_xe_32")
+    zsurf = -14.0
+    out = {}
+    for key in (".false.", ".true."):
+        w = os.path.join(tmp_path, "s" + key.strip("."))
+        os.makedirs(w)
+        write_mode2(os.path.join(w, "den.in"), d["psi0"], cfg["xmax"], cfg["ymax"], cfg["zmax"],
+                    d["rimp0"], d["vimp0"])
+        r = run_drive(w, namelist(cfg, niter=6, lsurf=key, lsurf3D=".false.", Zsurf=repr(zsurf)))
+        assert r.returncode == 0, r.stderr
+        m = re.search(r"FIRST ENERGY BALANCE:\s+TOTAL\s+energy \(He\) \.+:\s+(\S+) K", r.stdout)
+        assert m, r.stdout[-2000:]
+        out[key] = (float(m.group(1)), np.loadtxt(os.path.join(w, "rimp.out"), comments="#"))
+    assert np.array_equal(out[".false."][1], out[".true."][1])
+    nz, ny, nx = d["psi0"].shape
+    hz = 2.0 * abs(cfg["zmax"]) / nz
+    uz = np.array([min(3000.0, vsurf(0, 0.0, 0.0, -cfg["zmax"] + hz * k - zsurf)) for k in range(nz)])
+    dv = (2.0 * abs(cfg["xmax"]) / nx) * (2.0 * abs(cfg["ymax"]) / ny) * hz
+    rho = np.abs(d["psi0"]) ** 2
+    want = float((rho.sum(axis=(1, 2)) * uz).sum() * dv)
+    assert abs(want) > 1e-3  # the surface is close enough to matter
+    assert abs((out[".true."][0] - out[".false."][0]) - want) < 2e-6  # F18.6 output
+
+
+@pytest.mark.gpu
 def test_driver_restart_with_history_is_bitwise(tmp_path):
     """lhistory (an extension key): a run restarted from a checkpoint and its
     integrator history continues bit for bit as the uninterrupted run."""
